@@ -1,0 +1,169 @@
+/*
+ * tdfstat_hip.h -- C ABI of the MI355X (gfx950) implementation of the
+ * TDFstat coherent F-statistic search core (libtdfstat_hip.so).
+ *
+ * The library replaces the reference's job_core() path:
+ *   search/network/openmp/jobcore.c:193-630   job_core (per-sky stage +
+ *                                              spindown loop)
+ *   search/network/openmp/spinmod.c:17-538     spindown_modulation
+ *   search/network/openmp/settings.c:226-294   rogcvir / modvir
+ *   search/network/openmp/auxi.c:93-169        spline / splint / splintpad
+ *   search/network/openmp/auxi.c:336-363       FStat
+ *   search/network/openmp/init.c:710-755       plan_fftw (-> tdf_ctx_create)
+ *
+ * All arguments are plain C types and host pointers; the library owns its
+ * device memory and stream.  Every function returns TDF_OK (0) or a negative
+ * TDF_ERR_* code; tdf_strerror() gives the message.  The reference reports
+ * the same conditions by printing and calling exit(EXIT_FAILURE)
+ * (jobcore.c:452-455, 586-590); callers that want that behaviour print
+ * tdf_strerror() and exit.
+ *
+ * A context is not thread-safe; use one context per host thread.
+ */
+#ifndef TDFSTAT_HIP_H
+#define TDFSTAT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDF_OK 0
+#define TDF_ERR_ARG (-1)              /* invalid argument / size           */
+#define TDF_ERR_HIP (-2)              /* HIP runtime error                 */
+#define TDF_ERR_NOMEM (-3)            /* device or host allocation failed  */
+#define TDF_ERR_TRIGGER_OVERFLOW (-4) /* > bufsize/2 triggers in a spindown
+                                         (jobcore.c:586-590)               */
+#define TDF_ERR_UNSUPPORTED (-5)      /* size outside the supported range  */
+#define TDF_ERR_STATE (-6)            /* call order (e.g. no sky stage yet) */
+#define TDF_ERR_CAPACITY (-7)         /* caller buffer too small            */
+
+/* Arithmetic of the bulk path.
+ * TDF_PREC_FP32: FP32 resampling, weighting, FFT and F-statistic; the phase
+ *   arguments (het0*i + oms*shft, het1*i + spnd*tmp1), their range reduction
+ *   and the aa/bb normalisation sums stay FP64.
+ * TDF_PREC_FP64: everything in FP64 (the reference's PREC=DOUBLE build). */
+#define TDF_PREC_FP32 0
+#define TDF_PREC_FP64 1
+
+/* Sizes and constants of one data segment.  Field meanings follow
+ * Search_settings (struct.h:105-145) and settings.c:20-105. */
+typedef struct {
+    int N;            /* samples per detector                      */
+    int nfft;         /* 2^ceil(log2 N)                             */
+    int Ninterp;      /* interpftpad * nfft                         */
+    int nfftf;        /* fftpad * nfft                              */
+    int interpftpad;  /* 2 (settings.c:56)                          */
+    int nifo;         /* detectors, 1..8                            */
+    double oms;       /* 2 pi fpo dt                                */
+    double omr;       /* C_OMEGA_R dt                               */
+    double dt;
+    double sepsm, cepsm;
+    double Smin, Smax;
+    double M[16];     /* grid generating matrix (grid.bin)          */
+} tdf_geom;
+
+/* Per-call search options: the fields of Command_line_opts /
+ * Search_range / Search_settings that job_core reads. */
+typedef struct {
+    int gtype_allsky;  /* strcmp(opts->gtype,"allsky")==0 (jobcore.c:399)   */
+    float sst;         /* s_range->sst  (start spindown, non-allsky)        */
+    float spndr1;      /* s_range->spndr[1]                                 */
+    float sstep;       /* s_range->sstep                                    */
+    int fr0, fr1;      /* s_range->fr: F-statistic output bin range         */
+    double thr;        /* opts->thr                                         */
+    int fstat_norm;    /* 1 if opts->fstat_norm == "blocks_avg"             */
+    int nav;           /* FStat block length (NAVFSTAT, auxi.h:12), 2^k     */
+    int numlines;      /* sett->numlines_band                               */
+    const double *lines; /* sett->lines[numlines][2] (radians)              */
+    int bufsize;       /* sett->bufsize: floats per ffstat record           */
+} tdf_job_opts;
+
+/* Per-sky scalars (jobcore.c:242-262, 372-411). */
+typedef struct {
+    int visible;       /* 0: grid point outside the sky (jobcore.c:248)     */
+    double ra, dec;    /* sgnlt[3], sgnlt[2]                                */
+    double het0;
+    double aa, bb;     /* network normalisation sums / sig2                 */
+    float smin, smax;
+    int iss_size;      /* spindowns of this sky point                       */
+} tdf_sky;
+
+/* One spindown record (Trigger, struct.h:211-215); its ffstat pairs
+ * (f, F) are pairs[2*offset .. 2*(offset+ntrig)). */
+typedef struct {
+    float m, n, s;
+    float ra, dec, fdot;
+    int ntrig;
+    long long offset;
+} tdf_record;
+
+typedef struct tdf_ctx tdf_ctx;
+
+const char *tdf_strerror(int code);
+int tdf_device_count(int *n);
+
+/* Replaces plan_fftw (init.c:710-755): creates the device context, its
+ * FFT plans and work buffers.  batch = spindowns per launch (0 = auto). */
+int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out);
+int tdf_ctx_destroy(tdf_ctx *ctx);
+
+/* Uploads one detector's segment (ifo[det].sig.*: xDat float[N], DetSSB
+ * double[3N], sig2, cphir, sphir; amod c1..c9 from rogcvir). */
+int tdf_ctx_set_detector(tdf_ctx *ctx, int det, const float *xDat,
+                         const double *DetSSB, double sig2, double cphir,
+                         double sphir, const double amod[9]);
+
+/* Per-sky stage of job_core (jobcore.c:242-411): modvir, barycentric
+ * shift, FP64-phase demodulation, FFT interpolation, spline resampling and
+ * aa/bb sums for sky point (pm, mm, nn).  Results stay on the device. */
+int tdf_sky_stage(tdf_ctx *ctx, int pm, float mm, float nn,
+                  const tdf_job_opts *o, tdf_sky *sky);
+
+/* Spindown loop of job_core (jobcore.c:429-613) for spindowns
+ * iss_begin .. iss_begin+iss_count-1 of the current sky point.
+ * recs[iss_count] receive the records; pairs (capacity pair_cap pairs) the
+ * (f, F) trigger pairs in ascending frequency-bin order per record.
+ * *npairs returns the number of pairs written. */
+int tdf_spindown(tdf_ctx *ctx, int iss_begin, int iss_count,
+                 const tdf_job_opts *o, tdf_record *recs, float *pairs,
+                 long long pair_cap, long long *npairs);
+
+/* job_core (jobcore.c:193-630): per-sky stage + full spindown loop.
+ * *nrec = records written (= spindowns; 0 if the point is not visible). */
+int tdf_job_core(tdf_ctx *ctx, int pm, float mm, float nn,
+                 const tdf_job_opts *o, tdf_sky *sky, tdf_record *recs,
+                 int rec_cap, float *pairs, long long pair_cap, int *nrec,
+                 long long *npairs);
+
+/* ---- inspection (tests) ---- */
+/* per-sky arrays of detector det, converted to double: tmp1[N],
+ * shftf[N], xDatma/xDatmb[2N] interleaved complex (after /sig2).
+ * Any pointer may be NULL. */
+int tdf_debug_sky_arrays(tdf_ctx *ctx, int det, double *tmp1, double *shftf,
+                         double *xDatma, double *xDatmb);
+/* F-statistic (after optional FStat normalisation) of spindown
+ * iss of the last tdf_spindown call, natural bin order, length nfftf;
+ * bins outside [fr0, fr1] are 0. */
+int tdf_debug_fstat(tdf_ctx *ctx, int iss, double *F);
+
+/* ---- standalone building blocks on host buffers (tests) ---- */
+/* howmany unnormalised complex DFTs of length n (interleaved doubles),
+ * sign -1 forward (FFTW_FORWARD) / +1 backward; computed in prec. */
+int tdf_fft_c2c(const double *in, double *out, int n, int howmany, int sign,
+                int prec);
+
+/* ---- timing ---- */
+/* When enabled, every kernel launch of the spindown loop is bracketed
+ * by HIP events on the context's stream; tdf_kernel_times returns, per
+ * kernel slot, the summed milliseconds and launch count. */
+int tdf_profile_enable(tdf_ctx *ctx, int on);
+int tdf_kernel_times(tdf_ctx *ctx, int slot, double *ms, long long *launches,
+                     const char **name);
+int tdf_synchronize(tdf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,262 @@
+"""ctypes wrapper of the CPU oracle (liboracle: tdf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU
+baseline, never as the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtdf_oracle.so")
+PREC_DOUBLE, PREC_FLOAT = 0, 1
+
+
+class OrSett(C.Structure):
+    _fields_ = [("N", C.c_int), ("nfft", C.c_int), ("Ninterp", C.c_int), ("nfftf", C.c_int),
+                ("interpftpad", C.c_int), ("nifo", C.c_int),
+                ("oms", C.c_double), ("omr", C.c_double), ("dt", C.c_double), ("sepsm", C.c_double),
+                ("cepsm", C.c_double), ("Smin", C.c_double), ("Smax", C.c_double),
+                ("M", C.c_double * 16)]
+
+
+class OrDet(C.Structure):
+    _fields_ = [("xDat", C.POINTER(C.c_float)), ("DetSSB", C.POINTER(C.c_double)),
+                ("sig2", C.c_double), ("cphir", C.c_double), ("sphir", C.c_double),
+                ("c", C.c_double * 9)]
+
+
+class OrSky(C.Structure):
+    _fields_ = [("visible", C.c_int), ("ra", C.c_double), ("dec", C.c_double), ("het0", C.c_double),
+                ("aa", C.c_double), ("bb", C.c_double), ("aa_f", C.c_float), ("bb_f", C.c_float),
+                ("smin", C.c_float), ("smax", C.c_float), ("iss_size", C.c_int)]
+
+
+class OrOpts(C.Structure):
+    _fields_ = [("gtype_allsky", C.c_int), ("sst", C.c_float), ("spndr1", C.c_float),
+                ("sstep", C.c_float), ("fr0", C.c_int), ("fr1", C.c_int), ("thr", C.c_double),
+                ("fstat_norm", C.c_int), ("nav", C.c_int), ("numlines", C.c_int),
+                ("lines", C.POINTER(C.c_double)), ("bufsize", C.c_int)]
+
+
+_DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "or_lin2ast": (None, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _DP, _DP, _DP, _DP]),
+        "or_rogcvir": (None, [C.c_double, C.c_double, _DP]),
+        "or_modvir": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.POINTER(OrDet),
+                             _DP, _DP, _DP, _DP]),
+        "or_fft": (None, [_DP, C.c_int, C.c_int]),
+        "or_fftf": (None, [_FP, C.c_int, C.c_int]),
+        "or_spline": (None, [_DP, C.c_int, _DP]),
+        "or_splintpad": (None, [_DP, _DP, C.c_int, C.c_int, _DP]),
+        "or_fstat_norm": (C.c_double, [_DP, C.c_int, C.c_int]),
+        "or_sky_stage": (C.c_int, [C.POINTER(OrSett), C.POINTER(OrDet), C.c_int, C.c_float, C.c_float,
+                                   C.POINTER(OrOpts), C.POINTER(OrSky), _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
+        "or_spindown": (C.c_int, [C.POINTER(OrSett), C.POINTER(OrSky), C.c_int, C.c_float, C.c_float,
+                                  C.c_int, C.POINTER(OrOpts), _DP, _DP, _DP, _DP, C.c_int, _IP, _FP, _FP,
+                                  _FP]),
+        "or_triggers": (C.c_int, [_DP, C.POINTER(OrOpts), C.c_int, C.c_double, C.c_int, _IP, _FP, _FP]),
+        "or_job_core": (C.c_long, [C.POINTER(OrSett), C.POINTER(OrDet), C.c_int, C.c_float, C.c_float,
+                                   C.POINTER(OrOpts), C.c_int, C.c_int, C.c_int, C.POINTER(OrSky), C.c_long,
+                                   _IP, _IP, _FP, _FP, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def make_sett(sett):
+    s = OrSett()
+    s.N, s.nfft, s.Ninterp, s.nfftf = sett.N, sett.nfft, sett.Ninterp, sett.nfftf
+    s.interpftpad, s.nifo = sett.interpftpad, sett.nifo
+    s.oms, s.omr, s.dt = sett.oms, sett.omr, sett.dt
+    s.sepsm, s.cepsm, s.Smin, s.Smax = sett.sepsm, sett.cepsm, sett.Smin, sett.Smax
+    for k in range(16):
+        s.M[k] = float(sett.M[k])
+    return s
+
+
+class Dets:
+    """Keeps the numpy buffers alive while the C structs point into them."""
+
+    def __init__(self, dets):
+        self.keep = []
+        self.arr = (OrDet * len(dets))()
+        for k, d in enumerate(dets):
+            x = np.ascontiguousarray(d.xDat, dtype=np.float32)
+            s = np.ascontiguousarray(d.DetSSB, dtype=np.float64)
+            self.keep += [x, s]
+            self.arr[k].xDat = x.ctypes.data_as(_FP)
+            self.arr[k].DetSSB = s.ctypes.data_as(_DP)
+            self.arr[k].sig2 = d.sig2
+            self.arr[k].cphir = np.cos(d.phir)
+            self.arr[k].sphir = np.sin(d.phir)
+            for j in range(9):
+                self.arr[k].c[j] = float(d.amod[j])
+
+
+def make_opts(o):
+    """From tdfstat_amd.search.JobOpts."""
+    r = OrOpts()
+    r.gtype_allsky = int(o.gtype_allsky)
+    r.sst, r.spndr1, r.sstep = o.sst, o.spndr1, o.sstep
+    r.fr0, r.fr1, r.thr = o.fr0, o.fr1, o.thr
+    r.fstat_norm, r.nav = int(o.fstat_norm), o.nav
+    lines = np.ascontiguousarray(np.asarray(o.lines, dtype=np.float64).reshape(-1, 2)
+                                 if len(o.lines) else np.zeros((0, 2)))
+    r._keep = lines
+    r.numlines = lines.shape[0]
+    r.lines = _dp(lines) if r.numlines else _DP()
+    r.bufsize = o.bufsize
+    return r
+
+
+class Oracle:
+    """Per-segment oracle: sky stage + spindowns (FP64 = PREC=DOUBLE)."""
+
+    def __init__(self, sett, dets):
+        self.lib = load()
+        self.sett = sett
+        self.s = make_sett(sett)
+        self.d = Dets(dets)
+
+    def sky_stage(self, pm, mm, nn, opts):
+        N, nifo = self.sett.N, self.sett.nifo
+        o = make_opts(opts)
+        sky = OrSky()
+        xa = np.zeros(2 * nifo * N)
+        xb = np.zeros(2 * nifo * N)
+        t1 = np.zeros(nifo * N)
+        sh = np.zeros(nifo * N)
+        shf = np.zeros(nifo * N)
+        aa = np.zeros(nifo * N)
+        bb = np.zeros(nifo * N)
+        rc = self.lib.or_sky_stage(C.byref(self.s), self.d.arr, pm, mm, nn, C.byref(o), C.byref(sky),
+                                   _dp(xa), _dp(xb), _dp(t1), _dp(sh), _dp(shf), _dp(aa), _dp(bb))
+        assert rc == 0, rc
+        self.sky, self.mm, self.nn = sky, mm, nn
+        self.xa, self.xb, self.tmp1 = xa, xb, t1
+        return dict(sky=sky, xDatma=xa.view(np.complex128).reshape(nifo, N),
+                    xDatmb=xb.view(np.complex128).reshape(nifo, N), tmp1=t1.reshape(nifo, N),
+                    shft=sh.reshape(nifo, N), shftf=shf.reshape(nifo, N), aa=aa.reshape(nifo, N),
+                    bb=bb.reshape(nifo, N))
+
+    def spindown(self, iss, opts, prec=PREC_DOUBLE, max_trig=1 << 16):
+        o = make_opts(opts)
+        F = np.zeros(self.sett.nfftf)
+        ic = np.zeros(max_trig, dtype=np.int32)
+        tf = np.zeros(max_trig, dtype=np.float32)
+        tF = np.zeros(max_trig, dtype=np.float32)
+        rec = np.zeros(6, dtype=np.float32)
+        n = self.lib.or_spindown(C.byref(self.s), C.byref(self.sky), prec, self.mm, self.nn, iss, C.byref(o),
+                                 _dp(self.xa), _dp(self.xb), _dp(self.tmp1), _dp(F), max_trig,
+                                 ic.ctypes.data_as(_IP), tf.ctypes.data_as(_FP), tF.ctypes.data_as(_FP),
+                                 rec.ctypes.data_as(_FP))
+        return dict(F=F, ntrig=n, ic=ic[:max(n, 0)], f=tf[:max(n, 0)], Fv=tF[:max(n, 0)], rec=rec)
+
+    def job_core(self, pm, mm, nn, opts, prec=PREC_DOUBLE, iss_begin=0, iss_count=-1, nthreads=0,
+                 max_trig=1 << 20):
+        o = make_opts(opts)
+        sky = OrSky()
+        iss = np.zeros(max_trig, dtype=np.int32)
+        ic = np.zeros(max_trig, dtype=np.int32)
+        tf = np.zeros(max_trig, dtype=np.float32)
+        tF = np.zeros(max_trig, dtype=np.float32)
+        n = self.lib.or_job_core(C.byref(self.s), self.d.arr, pm, mm, nn, C.byref(o), prec, iss_begin,
+                                 iss_count, C.byref(sky), max_trig, iss.ctypes.data_as(_IP),
+                                 ic.ctypes.data_as(_IP), tf.ctypes.data_as(_FP), tF.ctypes.data_as(_FP),
+                                 nthreads)
+        k = max(int(n), 0)
+        return sky, int(n), dict(iss=iss[:k], ic=ic[:k], f=tf[:k], F=tF[:k])
+
+
+def fft(x, sign=-1):
+    lib = load()
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib.or_fft(_dp(a.view(np.float64)), a.size, sign)
+    return a
+
+
+def fftf(x, sign=-1):
+    lib = load()
+    a = np.ascontiguousarray(x, dtype=np.complex64).copy()
+    lib.or_fftf(a.view(np.float32).ctypes.data_as(_FP), a.size, sign)
+    return a
+
+
+def spline(y):
+    lib = load()
+    y = np.ascontiguousarray(y, dtype=np.complex128)
+    y2 = np.zeros_like(y)
+    lib.or_spline(_dp(y.view(np.float64)), y.size, _dp(y2.view(np.float64)))
+    return y2
+
+
+def splintpad(ya, shftf, N, interpftpad=2):
+    lib = load()
+    ya = np.ascontiguousarray(ya, dtype=np.complex128).copy()
+    shftf = np.ascontiguousarray(shftf, dtype=np.float64)
+    out = np.zeros(N, dtype=np.complex128)
+    lib.or_splintpad(_dp(ya.view(np.float64)), _dp(shftf), N, interpftpad, _dp(out.view(np.float64)))
+    return out
+
+
+def fstat_norm(F, nav):
+    lib = load()
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    lib.or_fstat_norm(_dp(F), F.size, nav)
+    return F
+
+
+def lin2ast(be1, be2, pm, sepsm, cepsm):
+    lib = load()
+    out = [C.c_double() for _ in range(4)]
+    lib.or_lin2ast(be1, be2, pm, sepsm, cepsm, *[C.byref(o) for o in out])
+    return tuple(o.value for o in out)
+
+
+def rogcvir(ephi, egam):
+    lib = load()
+    c = np.zeros(9)
+    lib.or_rogcvir(ephi, egam, _dp(c))
+    return c
+
+
+def triggers(F, opts, nfftf, sgnl0, max_trig=1 << 16):
+    """MAX_ALG 3 selection of the oracle on an F vector -> (n, ic, f, F)."""
+    lib = load()
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    o = make_opts(opts)
+    ic = np.zeros(max_trig, dtype=np.int32)
+    tf = np.zeros(max_trig, dtype=np.float32)
+    tF = np.zeros(max_trig, dtype=np.float32)
+    n = lib.or_triggers(_dp(F), C.byref(o), nfftf, sgnl0, max_trig, ic.ctypes.data_as(_IP),
+                        tf.ctypes.data_as(_FP), tF.ctypes.data_as(_FP))
+    k = max(n, 0)
+    return n, ic[:k], tf[:k], tF[:k]

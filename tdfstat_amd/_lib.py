@@ -1,0 +1,131 @@
+"""ctypes binding of libtdfstat_hip.so (include/tdfstat_hip.h).
+
+The shared library is built in-tree by ``tdfstat_amd.build`` (called from
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
+is no CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtdfstat_hip.so")
+
+TDF_OK = 0
+TDF_PREC_FP32 = 0
+TDF_PREC_FP64 = 1
+
+
+class TdfGeom(C.Structure):
+    _fields_ = [
+        ("N", C.c_int), ("nfft", C.c_int), ("Ninterp", C.c_int), ("nfftf", C.c_int),
+        ("interpftpad", C.c_int), ("nifo", C.c_int),
+        ("oms", C.c_double), ("omr", C.c_double), ("dt", C.c_double),
+        ("sepsm", C.c_double), ("cepsm", C.c_double),
+        ("Smin", C.c_double), ("Smax", C.c_double),
+        ("M", C.c_double * 16),
+    ]
+
+
+class TdfJobOpts(C.Structure):
+    _fields_ = [
+        ("gtype_allsky", C.c_int), ("sst", C.c_float), ("spndr1", C.c_float), ("sstep", C.c_float),
+        ("fr0", C.c_int), ("fr1", C.c_int), ("thr", C.c_double), ("fstat_norm", C.c_int),
+        ("nav", C.c_int), ("numlines", C.c_int), ("lines", C.POINTER(C.c_double)),
+        ("bufsize", C.c_int),
+    ]
+
+
+class TdfSky(C.Structure):
+    _fields_ = [
+        ("visible", C.c_int), ("ra", C.c_double), ("dec", C.c_double), ("het0", C.c_double),
+        ("aa", C.c_double), ("bb", C.c_double), ("smin", C.c_float), ("smax", C.c_float),
+        ("iss_size", C.c_int),
+    ]
+
+
+class TdfRecord(C.Structure):
+    _fields_ = [
+        ("m", C.c_float), ("n", C.c_float), ("s", C.c_float),
+        ("ra", C.c_float), ("dec", C.c_float), ("fdot", C.c_float),
+        ("ntrig", C.c_int), ("offset", C.c_longlong),
+    ]
+
+
+# every symbol declared in include/tdfstat_hip.h: name -> (restype, argtypes)
+_P = C.c_void_p
+_DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "tdf_strerror": (C.c_char_p, [C.c_int]),
+    "tdf_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "tdf_ctx_create": (C.c_int, [C.POINTER(TdfGeom), C.c_int, C.c_int, C.POINTER(_P)]),
+    "tdf_ctx_destroy": (C.c_int, [_P]),
+    "tdf_ctx_set_detector": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float), _DP, C.c_double,
+                                       C.c_double, C.c_double, _DP]),
+    "tdf_sky_stage": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.POINTER(TdfJobOpts),
+                                C.POINTER(TdfSky)]),
+    "tdf_spindown": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(TdfJobOpts), C.POINTER(TdfRecord),
+                               C.POINTER(C.c_float), C.c_longlong, C.POINTER(C.c_longlong)]),
+    "tdf_job_core": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.POINTER(TdfJobOpts),
+                               C.POINTER(TdfSky), C.POINTER(TdfRecord), C.c_int,
+                               C.POINTER(C.c_float), C.c_longlong, C.POINTER(C.c_int),
+                               C.POINTER(C.c_longlong)]),
+    "tdf_debug_sky_arrays": (C.c_int, [_P, C.c_int, _DP, _DP, _DP, _DP]),
+    "tdf_debug_fstat": (C.c_int, [_P, C.c_int, _DP]),
+    "tdf_fft_c2c": (C.c_int, [_DP, _DP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "tdf_profile_enable": (C.c_int, [_P, C.c_int]),
+    "tdf_kernel_times": (C.c_int, [_P, C.c_int, _DP, C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_char_p)]),
+    "tdf_synchronize": (C.c_int, [_P]),
+}
+
+_lib = None
+
+
+class TdfError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        msg = load().tdf_strerror(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{where}: {msg} (code {code})")
+
+
+def load():
+    """Load libtdfstat_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, where):
+    if rc != TDF_OK:
+        raise TdfError(rc, where)
+
+
+def dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def fft_c2c(x, sign=-1, prec=TDF_PREC_FP64):
+    """Unnormalised DFT of each row of complex array x on the GPU."""
+    lib = load()
+    x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.complex128)
+    howmany, n = x.shape
+    out = np.empty_like(x)
+    check(lib.tdf_fft_c2c(dptr(x.view(np.float64)), dptr(out.view(np.float64)), n, howmany,
+                          sign, prec), "tdf_fft_c2c")
+    return out

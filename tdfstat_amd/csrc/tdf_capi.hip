@@ -1,0 +1,797 @@
+// C ABI of libtdfstat_hip.so (declared in include/tdfstat_hip.h).
+//
+// Host side of job_core (search/network/openmp/jobcore.c:193-630): the
+// per-sky scalars (grid point -> sky, het0, spindown range), the per-template
+// scalars (spnd, het1, sgnl0, record fields) and the batch loop over
+// spindowns.  The bulk arithmetic runs in the kernels of tdf_kernels.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/tdfstat_hip.h"
+#include "tdf_internal.h"
+
+using namespace tdf;
+
+// The per-sky and per-template scalars (het0, het1, spnd, sgnl0, f_) must
+// round exactly like the reference's separate multiply/add operations.
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int NSLOT_PROF = 16;
+enum {
+    KS_SKY_PREP,
+    KS_SKY_REDUCE,
+    KS_SKY_FFT,
+    KS_SKY_TRANSPOSE,
+    KS_SKY_IFFT,
+    KS_SKY_TRANSPOSE2,
+    KS_SPLINE,
+    KS_SPLINT,
+    KS_SPIN1,
+    KS_SPIN2,
+    KS_FSTAT,
+    KS_TRIG,
+    KS_COUNT
+};
+const char *kslot_names[KS_COUNT] = {"sky_prep",   "sky_reduce", "sky_fft_fwd", "sky_transpose_move",
+                                     "sky_fft_inv", "sky_transpose", "spline_coef", "splint",
+                                     "spin_pass1_demod_fft", "spin_pass2_fft_fstat", "fstat_blocks",
+                                     "triggers"};
+
+int ilog2i(long long n)
+{
+    int l = 0;
+    while ((1LL << l) < n) l++;
+    return l;
+}
+bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+struct tdf_ctx {
+    tdf_geom g;
+    int prec;    // TDF_PREC_*
+    int batch;
+    size_t csz;  // sizeof complex element
+    size_t rsz;  // sizeof real element
+    hipStream_t stream = nullptr;
+    // twiddle tables
+    std::vector<void *> allocs;
+    const void *twL[2][16] = {};
+    FftPlan p_fwd{}, p_inv{}, p_spin{};
+    // detector data
+    float *xDat = nullptr;
+    double *DetSSB = nullptr;
+    DetParams *detp = nullptr;
+    double *sig2_d = nullptr;
+    std::vector<DetParams> detp_h;
+    std::vector<double> ssb0_h;  // first DetSSB triple per detector
+    std::vector<int> det_set;
+    // per-sky buffers
+    void *xab = nullptr, *Zsky = nullptr, *XT = nullptr, *Ymv = nullptr;
+    double *shftf = nullptr, *tmp1 = nullptr, *partials = nullptr, *sky_scal = nullptr;
+    void *xm = nullptr;
+    double *taps = nullptr;
+    int prep_nblk = 0;
+    bool sky_valid = false;
+    tdf_sky sky{};
+    float mm = 0, nn = 0;
+    int pm = 0;
+    // spindown batch buffers
+    void *Z = nullptr, *FT = nullptr, *mu = nullptr;
+    int nblk_cap = 0;
+    TmplDesc *desc_d = nullptr, *desc_h = nullptr;
+    TrigEntry *trig_d = nullptr, *trig_h = nullptr;
+    int *cnt_d = nullptr, *cnt_h = nullptr;
+    long long trig_cap = 0;
+    double *lines_d = nullptr;
+    int lines_cap = 0;
+    int last_batch_n = 0, last_batch_iss0 = 0;
+    int last_fr0 = 0, last_fr1 = -1;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_slot;
+    size_t ev_used = 0;
+    double kms[KS_COUNT] = {};
+    long long kcount[KS_COUNT] = {};
+};
+
+#define HIPCHK(x)                                                                            \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) {                                                              \
+            fprintf(stderr, "[tdfstat_hip] %s failed: %s\n", #x, hipGetErrorString(e_));     \
+            return TDF_ERR_HIP;                                                              \
+        }                                                                                    \
+    } while (0)
+
+namespace {
+
+int dmalloc(tdf_ctx *c, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return TDF_ERR_NOMEM;
+    }
+    c->allocs.push_back(*p);
+    return TDF_OK;
+}
+
+// Event bracketing of one launch (profiling mode).
+struct Prof {
+    tdf_ctx *c;
+    int slot;
+    size_t idx;
+    Prof(tdf_ctx *c_, int s) : c(c_), slot(s), idx(0)
+    {
+        if (!c->prof) return;
+        if (c->ev_used + 2 > c->ev.size()) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            c->ev.push_back(a);
+            c->ev.push_back(b);
+            c->ev_slot.push_back(s);
+            c->ev_slot.push_back(s);
+        }
+        idx = c->ev_used;
+        c->ev_slot[idx / 2 * 2] = s;
+        (void)hipEventRecord(c->ev[idx], c->stream);
+    }
+    ~Prof()
+    {
+        if (!c->prof) return;
+        (void)hipEventRecord(c->ev[idx + 1], c->stream);
+        c->ev_used += 2;
+    }
+};
+
+void prof_collect(tdf_ctx *c)
+{
+    if (!c->prof) return;
+    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) {
+            c->kms[c->ev_slot[i]] += ms;
+            c->kcount[c->ev_slot[i]] += 1;
+        }
+    }
+    c->ev_used = 0;
+}
+
+// W_L^m = exp(-2 pi i m / L), m < L, in precision T
+template <typename T>
+int make_table(tdf_ctx *c, long long L, long long count, long long stride, const void **out)
+{
+    std::vector<T> h(2 * count);
+    for (long long m = 0; m < count; m++) {
+        // exponent m*stride modulo L, angle computed from the reduced index
+        long long e = (m * stride) % L;
+        double s, co;
+        sincos(-2.0 * M_PI * (double)e / (double)L, &s, &co);
+        h[2 * m] = (T)co;
+        h[2 * m + 1] = (T)s;
+    }
+    void *d;
+    int rc = dmalloc(c, &d, sizeof(T) * h.size());
+    if (rc) return rc;
+    if (hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return TDF_ERR_HIP;
+    *out = d;
+    return TDF_OK;
+}
+
+int make_plan(tdf_ctx *c, int n, FftPlan *p)
+{
+    if (!is_pow2(n) || n < 4096) return TDF_ERR_UNSUPPORTED;
+    const int elems_half = c->prec == TDF_PREC_FP64 ? 1024 : 2048;
+    int n1 = std::min(elems_half, n / 64);
+    int n2 = n / n1;
+    if (n2 > elems_half) return TDF_ERR_UNSUPPORTED;
+    p->n = n;
+    p->n1 = n1;
+    p->n2 = n2;
+    p->l1 = ilog2i(n1);
+    p->l2 = ilog2i(n2);
+    for (int d = 0; d < 2; d++)
+        for (int l = 0; l < 16; l++) p->twL[d][l] = c->twL[d][l];
+    int rc;
+    const int dd = c->prec == TDF_PREC_FP64 ? 1 : 0;
+    if (dd) {
+        if ((rc = make_table<double>(c, n, TW_LO, 1, &p->tw_lo[1]))) return rc;
+        if ((rc = make_table<double>(c, n, std::max(1, n / TW_LO), TW_LO, &p->tw_hi[1]))) return rc;
+    } else {
+        if ((rc = make_table<float>(c, n, TW_LO, 1, &p->tw_lo[0]))) return rc;
+        if ((rc = make_table<float>(c, n, std::max(1, n / TW_LO), TW_LO, &p->tw_hi[0]))) return rc;
+    }
+    return TDF_OK;
+}
+
+template <typename T>
+int run_sky_stage(tdf_ctx *c, const SkyPrepArgs &a)
+{
+    const tdf_geom &g = c->g;
+    const int nifo = g.nifo;
+    hipStream_t st = c->stream;
+    {
+        Prof pr(c, KS_SKY_PREP);
+        HIPCHK(launch_sky_prep<T>(a, c->xab, c->prep_nblk, nifo, st));
+    }
+    {
+        Prof pr(c, KS_SKY_REDUCE);
+        HIPCHK(launch_sky_reduce(c->partials, c->prep_nblk, nifo, c->sig2_d, c->sky_scal, st));
+    }
+    // forward FFT of length nfft, result column-major in XT
+    {
+        Prof pr(c, KS_SKY_FFT);
+        HIPCHK(launch_fft_natural_to_T<T>(c->p_fwd, c->xab, c->Zsky, c->XT, nifo, -1, 0, 0, 1.0, st));
+    }
+    // natural order + spectrum move into Ymv (jobcore.c:340-350)
+    const int nyqst = g.nfft / 2 + 1;
+    {
+        Prof pr(c, KS_SKY_TRANSPOSE);
+        HIPCHK(launch_transpose<T>(c->p_fwd, c->XT, c->Ymv, 2 * nifo, (size_t)g.nfft, (size_t)g.Ninterp, nyqst,
+                                   g.Ninterp - g.nfft, g.nfft, st));
+    }
+    // inverse FFT of length Ninterp, scaled by interpftpad/Ninterp
+    const double ft = (double)g.interpftpad / g.Ninterp;
+    {
+        Prof pr(c, KS_SKY_IFFT);
+        HIPCHK(launch_fft_natural_to_T<T>(c->p_inv, c->Ymv, c->Zsky, c->XT, nifo, +1, nyqst,
+                                          nyqst + g.Ninterp - g.nfft, ft, st));
+    }
+    const int nnodes = g.interpftpad * g.N;
+    {
+        Prof pr(c, KS_SKY_TRANSPOSE2);
+        HIPCHK(launch_transpose<T>(c->p_inv, c->XT, c->Ymv, 2 * nifo, (size_t)g.Ninterp, (size_t)g.Ninterp,
+                                   g.Ninterp, 0, std::min(g.Ninterp, nnodes + 1), st));
+    }
+    {
+        Prof pr(c, KS_SPLINE);
+        HIPCHK(launch_spline<T>(c->Ymv, c->Zsky, nnodes, (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, c->taps,
+                                st));
+    }
+    {
+        Prof pr(c, KS_SPLINT);
+        HIPCHK(launch_splint<T>(c->Ymv, c->Zsky, c->shftf, c->sig2_d, c->xm, g.N, nnodes, g.interpftpad,
+                                (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, st));
+    }
+    return TDF_OK;
+}
+
+int check_opts(tdf_ctx *c, const tdf_job_opts *o)
+{
+    if (!o) return TDF_ERR_ARG;
+    if (o->fr0 < 0 || o->fr1 >= c->g.nfftf || o->fr1 < o->fr0) return TDF_ERR_ARG;
+    if (o->fstat_norm) {
+        const int tb = c->prec == TDF_PREC_FP64 ? TrigCfg<double>::TB : TrigCfg<float>::TB;
+        if (!is_pow2(o->nav) || o->nav > tb) return TDF_ERR_UNSUPPORTED;
+    }
+    if (o->numlines < 0 || (o->numlines > 0 && !o->lines)) return TDF_ERR_ARG;
+    if (o->bufsize < 0) return TDF_ERR_ARG;
+    return TDF_OK;
+}
+
+template <typename T>
+int run_spindown_batch(tdf_ctx *c, int nslots, const tdf_job_opts *o)
+{
+    hipStream_t st = c->stream;
+    const FftPlan &p = c->p_spin;
+    const tdf_geom &g = c->g;
+    HIPCHK(hipMemcpyAsync(c->desc_d, c->desc_h, sizeof(TmplDesc) * nslots, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(c->cnt_d, 0, sizeof(int) * (1 + c->batch), st));
+    SpinArgs sa{c->xm, c->tmp1, c->desc_d, c->Z, c->FT, c->sky_scal, g.N, g.nifo, 0, nslots, o->fr0, o->fr1};
+    {
+        Prof pr(c, KS_SPIN1);
+        HIPCHK(launch_spin_pass1<T>(p, sa, st));
+    }
+    {
+        Prof pr(c, KS_SPIN2);
+        HIPCHK(launch_spin_pass2<T>(p, sa, st));
+    }
+    const int nblk = o->fstat_norm ? (o->fr1 - o->fr0) / o->nav : 0;
+    if (o->fstat_norm && nblk > 0) {
+        FstatArgs fa{p.n, p.n1, p.n2, o->fr0, o->fr1, nblk, o->nav, c->nblk_cap};
+        Prof pr(c, KS_FSTAT);
+        HIPCHK(launch_fstat_blocks<T>(c->FT, c->mu, fa, nslots, st));
+    }
+    TrigArgs ta{};
+    ta.n = p.n;
+    ta.n1 = p.n1;
+    ta.n2 = p.n2;
+    ta.nfftf = g.nfftf;
+    ta.fr0 = o->fr0;
+    ta.fr1 = o->fr1;
+    ta.thr = o->thr;
+    ta.fstat_norm = (o->fstat_norm && nblk > 0) ? 1 : 0;
+    ta.nblk = nblk;
+    ta.nav = o->nav > 0 ? o->nav : 1;
+    ta.nblk_cap = c->nblk_cap;
+    ta.numlines = o->numlines;
+    ta.cap = (int)std::min<long long>(c->trig_cap, 0x7fffffff);
+    ta.lines = c->lines_d;
+    ta.desc = c->desc_d;
+    {
+        Prof pr(c, KS_TRIG);
+        HIPCHK(launch_triggers<T>(c->FT, c->mu, ta, nslots, c->trig_d, c->cnt_d, st));
+    }
+    HIPCHK(hipMemcpyAsync(c->cnt_h, c->cnt_d, sizeof(int) * (1 + nslots), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const long long ntot = c->cnt_h[0];
+    if (ntot > 0) {
+        const long long ncp = std::min(ntot, c->trig_cap);
+        HIPCHK(hipMemcpyAsync(c->trig_h, c->trig_d, sizeof(TrigEntry) * ncp, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    prof_collect(c);
+    return TDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tdf_strerror(int code)
+{
+    switch (code) {
+    case TDF_OK: return "ok";
+    case TDF_ERR_ARG: return "invalid argument";
+    case TDF_ERR_HIP: return "HIP runtime error";
+    case TDF_ERR_NOMEM: return "cannot allocate memory";
+    case TDF_ERR_TRIGGER_OVERFLOW: return "Too many triggers! (more than bufsize/2 in one spindown)";
+    case TDF_ERR_UNSUPPORTED: return "size or option outside the supported range";
+    case TDF_ERR_STATE: return "call order: no per-sky stage computed";
+    case TDF_ERR_CAPACITY: return "caller buffer too small";
+    default: return "unknown error";
+    }
+}
+
+int tdf_device_count(int *n)
+{
+    if (!n) return TDF_ERR_ARG;
+    HIPCHK(hipGetDeviceCount(n));
+    return TDF_OK;
+}
+
+int tdf_ctx_destroy(tdf_ctx *c)
+{
+    if (!c) return TDF_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->desc_h) (void)hipHostFree(c->desc_h);
+    if (c->trig_h) (void)hipHostFree(c->trig_h);
+    if (c->cnt_h) (void)hipHostFree(c->cnt_h);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TDF_OK;
+}
+
+int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
+{
+    if (!g || !out) return TDF_ERR_ARG;
+    *out = nullptr;
+    if (prec != TDF_PREC_FP32 && prec != TDF_PREC_FP64) return TDF_ERR_ARG;
+    if (g->nifo < 1 || g->nifo > MAXDET || g->N < 1) return TDF_ERR_ARG;
+    if (!is_pow2(g->nfft) || g->nfft < g->N || g->interpftpad != 2 || g->Ninterp != 2 * g->nfft)
+        return TDF_ERR_UNSUPPORTED;
+    if (!is_pow2(g->nfftf) || g->nfftf < g->nfft) return TDF_ERR_UNSUPPORTED;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TDF_ERR_HIP;
+    tdf_ctx *c = new tdf_ctx();
+    c->g = *g;
+    c->prec = prec;
+    c->csz = prec == TDF_PREC_FP64 ? 16 : 8;
+    c->rsz = prec == TDF_PREC_FP64 ? 8 : 4;
+    c->batch = batch > 0 ? batch : (g->nfftf >= (1 << 19) ? 64 : 128);
+    int rc = TDF_OK;
+#define TRY(x)                    \
+    do {                          \
+        rc = (x);                 \
+        if (rc) {                 \
+            tdf_ctx_destroy(c);   \
+            return rc;            \
+        }                         \
+    } while (0)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return TDF_ERR_HIP;
+    }
+    // per-length tables W_L for L = 64..2048
+    for (int l = 6; l <= 11; l++) {
+        if (prec == TDF_PREC_FP64) TRY(make_table<double>(c, 1LL << l, 1LL << l, 1, &c->twL[1][l]));
+        else TRY(make_table<float>(c, 1LL << l, 1LL << l, 1, &c->twL[0][l]));
+    }
+    TRY(make_plan(c, g->nfft, &c->p_fwd));
+    TRY(make_plan(c, g->Ninterp, &c->p_inv));
+    TRY(make_plan(c, g->nfftf, &c->p_spin));
+    const int nifo = g->nifo, N = g->N;
+    const size_t cs = c->csz;
+    TRY(dmalloc(c, (void **)&c->xDat, sizeof(float) * nifo * (size_t)N));
+    TRY(dmalloc(c, (void **)&c->DetSSB, sizeof(double) * 3 * nifo * (size_t)N));
+    TRY(dmalloc(c, (void **)&c->detp, sizeof(DetParams) * nifo));
+    TRY(dmalloc(c, (void **)&c->sig2_d, sizeof(double) * nifo));
+    c->detp_h.assign(nifo, DetParams{});
+    c->ssb0_h.assign(3 * nifo, 0.);
+    c->det_set.assign(nifo, 0);
+    TRY(dmalloc(c, &c->xab, cs * 2 * nifo * (size_t)g->nfft));
+    TRY(dmalloc(c, &c->Zsky, cs * 2 * nifo * (size_t)g->Ninterp));
+    TRY(dmalloc(c, &c->XT, cs * 2 * nifo * (size_t)g->Ninterp));
+    TRY(dmalloc(c, &c->Ymv, cs * 2 * nifo * (size_t)g->Ninterp));
+    TRY(dmalloc(c, (void **)&c->shftf, sizeof(double) * nifo * (size_t)N));
+    TRY(dmalloc(c, (void **)&c->tmp1, sizeof(double) * nifo * (size_t)N));
+    TRY(dmalloc(c, &c->xm, cs * 2 * nifo * (size_t)N));
+    c->prep_nblk = (g->nfft + 255) / 256;
+    TRY(dmalloc(c, (void **)&c->partials, sizeof(double) * 2 * nifo * (size_t)c->prep_nblk));
+    TRY(dmalloc(c, (void **)&c->sky_scal, sizeof(double) * 4));
+    {
+        // spline taps: 1/(2 sqrt 3) (-r)^k, k <= K, then (-r)^i, i < BK
+        const double r = 2. - sqrt(3.);
+        const int K = prec == TDF_PREC_FP64 ? SplineCfg<double>::K : SplineCfg<float>::K;
+        const int BK = prec == TDF_PREC_FP64 ? SplineCfg<double>::BK : SplineCfg<float>::BK;
+        std::vector<double> t(K + 1 + BK);
+        double pw = 1.;
+        for (int k = 0; k <= K; k++, pw *= -r) t[k] = pw / (2. * sqrt(3.));
+        pw = 1.;
+        for (int i = 0; i < BK; i++, pw *= -r) t[K + 1 + i] = pw;
+        TRY(dmalloc(c, (void **)&c->taps, sizeof(double) * t.size()));
+        if (hipMemcpy(c->taps, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice) != hipSuccess)
+            TRY(TDF_ERR_HIP);
+    }
+    const size_t B = (size_t)c->batch;
+    TRY(dmalloc(c, &c->Z, cs * 2 * B * (size_t)g->nfftf));
+    TRY(dmalloc(c, &c->FT, c->rsz * B * (size_t)g->nfftf));
+    c->nblk_cap = g->nfftf / 16 + 1;  // nav >= 16 in practice; checked per call
+    TRY(dmalloc(c, &c->mu, c->rsz * B * (size_t)c->nblk_cap));
+    TRY(dmalloc(c, (void **)&c->desc_d, sizeof(TmplDesc) * B));
+    if (hipHostMalloc((void **)&c->desc_h, sizeof(TmplDesc) * B, 0) != hipSuccess) TRY(TDF_ERR_NOMEM);
+    TRY(dmalloc(c, (void **)&c->cnt_d, sizeof(int) * (1 + B)));
+    if (hipHostMalloc((void **)&c->cnt_h, sizeof(int) * (1 + B), 0) != hipSuccess) TRY(TDF_ERR_NOMEM);
+    // every record may hold up to nfftf/2 pairs (bufsize <= nfftf)
+    c->trig_cap = std::min<long long>((long long)B * (g->nfftf / 2 + 1), 1LL << 24);
+    TRY(dmalloc(c, (void **)&c->trig_d, sizeof(TrigEntry) * c->trig_cap));
+    if (hipHostMalloc((void **)&c->trig_h, sizeof(TrigEntry) * c->trig_cap, 0) != hipSuccess) TRY(TDF_ERR_NOMEM);
+    c->lines_cap = 512;
+    TRY(dmalloc(c, (void **)&c->lines_d, sizeof(double) * 2 * c->lines_cap));
+#undef TRY
+    *out = c;
+    return TDF_OK;
+}
+
+int tdf_ctx_set_detector(tdf_ctx *c, int det, const float *xDat, const double *DetSSB, double sig2, double cphir,
+                         double sphir, const double amod[9])
+{
+    if (!c || det < 0 || det >= c->g.nifo || !xDat || !DetSSB || !amod) return TDF_ERR_ARG;
+    const size_t N = c->g.N;
+    HIPCHK(hipMemcpy(c->xDat + det * N, xDat, sizeof(float) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->DetSSB + det * 3 * N, DetSSB, sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+    DetParams &dp = c->detp_h[det];
+    for (int k = 0; k < 9; k++) dp.c[k] = amod[k];
+    dp.sig2 = sig2;
+    dp.cphir = cphir;
+    dp.sphir = sphir;
+    HIPCHK(hipMemcpy(c->detp + det, &dp, sizeof(DetParams), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->sig2_d + det, &sig2, sizeof(double), hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; k++) c->ssb0_h[3 * det + k] = DetSSB[k];
+    c->det_set[det] = 1;
+    c->sky_valid = false;
+    return TDF_OK;
+}
+
+int tdf_sky_stage(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o, tdf_sky *sky)
+{
+    if (!c || !sky) return TDF_ERR_ARG;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    for (int d = 0; d < c->g.nifo; d++)
+        if (!c->det_set[d]) return TDF_ERR_STATE;
+    const tdf_geom &g = c->g;
+    const double *M = g.M;
+    tdf_sky s{};
+    c->sky_valid = false;
+    // jobcore.c:242-248
+    const double al1 = nn * M[10] + mm * M[14];
+    const double al2 = nn * M[11] + mm * M[15];
+    const double cof = g.oms;
+    if ((al1 * al1 + al2 * al2) / (cof * cof) > 1.) {
+        s.visible = 0;
+        *sky = s;
+        c->sky = s;
+        return TDF_OK;
+    }
+    s.visible = 1;
+    // lin2ast (auxi.c:22-29)
+    const double be1 = al1 / cof, be2 = al2 / cof;
+    const double sindel = be1 * g.sepsm - (2 * pm - 3) * sqrt(1. - be1 * be1 - be2 * be2) * g.cepsm;
+    const double cosdel = sqrt(1. - sindel * sindel);
+    const double sinal = (be1 - g.sepsm * sindel) / (g.cepsm * cosdel);
+    const double cosal = be2 / cosdel;
+    s.dec = asin(sindel);
+    s.ra = fmod(atan2(sinal, cosal) + 2. * M_PI, 2. * M_PI);
+    s.het0 = fmod(nn * M[8] + mm * M[12], M[0]);
+    SkyPrepArgs a{};
+    a.xDat = c->xDat;
+    a.DetSSB = c->DetSSB;
+    a.det = c->detp;
+    a.shftf = c->shftf;
+    a.tmp1 = c->tmp1;
+    a.partials = c->partials;
+    a.N = g.N;
+    a.nfft = g.nfft;
+    a.omr = g.omr;
+    a.oms = g.oms;
+    a.het0 = s.het0;
+    a.ns0 = cosal * cosdel;
+    a.ns1 = sinal * cosdel;
+    a.ns2 = sindel;
+    a.c2d = cosdel * cosdel;
+    a.c2sd = sindel * cosdel;
+    a.sindel = sindel;
+    a.cosdel = cosdel;
+    for (int d = 0; d < g.nifo; d++) {
+        const DetParams &dp = c->detp_h[d];
+        a.shft1[d] = a.ns0 * c->ssb0_h[3 * d] + a.ns1 * c->ssb0_h[3 * d + 1] + a.ns2 * c->ssb0_h[3 * d + 2];
+        a.cosalfr[d] = cosal * dp.cphir + sinal * dp.sphir;
+        a.sinalfr[d] = sinal * dp.cphir - cosal * dp.sphir;
+    }
+    rc = c->prec == TDF_PREC_FP64 ? run_sky_stage<double>(c, a) : run_sky_stage<float>(c, a);
+    if (rc) return rc;
+    double scal[2];
+    HIPCHK(hipMemcpyAsync(scal, c->sky_scal, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+    s.aa = scal[0];
+    s.bb = scal[1];
+    // spindown range (jobcore.c:399-411)
+    float smin = o->sst, smax = o->spndr1;
+    if (o->gtype_allsky) {
+        smin = trunc((g.Smin - nn * M[9] - mm * M[13]) / M[5]);
+        smax = trunc(-(nn * M[9] + mm * M[13] + g.Smax) / M[5]);
+        if (smin > smax) {
+            smin = smin + smax;
+            smax = smin - smax;
+            smin = smin - smax;
+        }
+    }
+    s.smin = smin;
+    s.smax = smax;
+    s.iss_size = (int)((smax - smin) / o->sstep + 1);
+    if (s.iss_size < 0) s.iss_size = 0;
+    *sky = s;
+    c->sky = s;
+    c->pm = pm;
+    c->mm = mm;
+    c->nn = nn;
+    c->sky_valid = true;
+    return TDF_OK;
+}
+
+int tdf_spindown(tdf_ctx *c, int iss_begin, int iss_count, const tdf_job_opts *o, tdf_record *recs, float *pairs,
+                 long long pair_cap, long long *npairs)
+{
+    if (!c || !recs || iss_begin < 0 || iss_count < 0 || (pair_cap > 0 && !pairs)) return TDF_ERR_ARG;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    if (!c->sky_valid || !c->sky.visible) return TDF_ERR_STATE;
+    if (o->numlines > c->lines_cap) return TDF_ERR_UNSUPPORTED;
+    if (o->fstat_norm && (o->fr1 - o->fr0) / o->nav > c->nblk_cap) return TDF_ERR_UNSUPPORTED;
+    const tdf_geom &g = c->g;
+    const double *M = g.M;
+    const float mm = c->mm, nn = c->nn;
+    if (o->numlines > 0)
+        HIPCHK(hipMemcpyAsync(c->lines_d, o->lines, sizeof(double) * 2 * o->numlines, hipMemcpyHostToDevice,
+                              c->stream));
+    long long np = 0;
+    const int maxtrig = o->bufsize / 2;
+    std::vector<std::vector<std::pair<int, std::pair<float, float>>>> per;
+    for (int b0 = 0; b0 < iss_count; b0 += c->batch) {
+        const int nslots = std::min(c->batch, iss_count - b0);
+        for (int k = 0; k < nslots; k++) {
+            const int iss = iss_begin + b0 + k;
+            // jobcore.c:430-468 (DOUBLE build arithmetic)
+            const float ss = c->sky.smin + iss * o->sstep;
+            const double spnd = ss * M[5] + nn * M[9] + mm * M[13];
+            double het1 = fmod(ss * M[4], M[0]);
+            if (het1 < 0) het1 += M[0];
+            c->desc_h[k] = TmplDesc{het1, spnd, c->sky.het0 + het1};
+            tdf_record &r = recs[b0 + k];
+            r.m = mm;
+            r.n = nn;
+            r.s = ss;
+            r.ra = (float)c->sky.ra;
+            r.dec = (float)c->sky.dec;
+            r.fdot = (float)(spnd / (M_PI * g.dt * g.dt));
+            r.ntrig = 0;
+            r.offset = 0;
+        }
+        rc = c->prec == TDF_PREC_FP64 ? run_spindown_batch<double>(c, nslots, o)
+                                      : run_spindown_batch<float>(c, nslots, o);
+        if (rc) return rc;
+        c->last_batch_n = nslots;
+        c->last_batch_iss0 = iss_begin + b0;
+        c->last_fr0 = o->fr0;
+        c->last_fr1 = o->fr1;
+        for (int k = 0; k < nslots; k++)
+            if (c->cnt_h[1 + k] > maxtrig) return TDF_ERR_TRIGGER_OVERFLOW;
+        const long long ntot = c->cnt_h[0];
+        if (ntot > c->trig_cap) return TDF_ERR_TRIGGER_OVERFLOW;
+        per.assign(nslots, {});
+        for (long long t = 0; t < ntot; t++) {
+            const TrigEntry &e = c->trig_h[t];
+            per[e.slot].push_back({e.ic, {e.f, e.F}});
+        }
+        for (int k = 0; k < nslots; k++) {
+            auto &v = per[k];
+            std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+            tdf_record &r = recs[b0 + k];
+            r.ntrig = (int)v.size();
+            r.offset = np;
+            if (np + (long long)v.size() > pair_cap) return TDF_ERR_CAPACITY;
+            for (auto &e : v) {
+                pairs[2 * np] = e.second.first;
+                pairs[2 * np + 1] = e.second.second;
+                np++;
+            }
+        }
+    }
+    if (npairs) *npairs = np;
+    return TDF_OK;
+}
+
+int tdf_job_core(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o, tdf_sky *sky, tdf_record *recs,
+                 int rec_cap, float *pairs, long long pair_cap, int *nrec, long long *npairs)
+{
+    if (!c || !sky || !nrec) return TDF_ERR_ARG;
+    *nrec = 0;
+    if (npairs) *npairs = 0;
+    int rc = tdf_sky_stage(c, pm, mm, nn, o, sky);
+    if (rc) return rc;
+    if (!sky->visible || sky->iss_size <= 0) return TDF_OK;
+    if (sky->iss_size > rec_cap) return TDF_ERR_CAPACITY;
+    rc = tdf_spindown(c, 0, sky->iss_size, o, recs, pairs, pair_cap, npairs);
+    if (rc) return rc;
+    *nrec = sky->iss_size;
+    return TDF_OK;
+}
+
+int tdf_debug_sky_arrays(tdf_ctx *c, int det, double *tmp1, double *shftf, double *xDatma, double *xDatmb)
+{
+    if (!c || det < 0 || det >= c->g.nifo) return TDF_ERR_ARG;
+    if (!c->sky_valid) return TDF_ERR_STATE;
+    const size_t N = c->g.N;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (tmp1) HIPCHK(hipMemcpy(tmp1, c->tmp1 + det * N, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (shftf) HIPCHK(hipMemcpy(shftf, c->shftf + det * N, sizeof(double) * N, hipMemcpyDeviceToHost));
+    for (int w = 0; w < 2; w++) {
+        double *dst = w == 0 ? xDatma : xDatmb;
+        if (!dst) continue;
+        const char *src = (const char *)c->xm + (2 * det + w) * N * c->csz;
+        if (c->prec == TDF_PREC_FP64) {
+            HIPCHK(hipMemcpy(dst, src, 16 * N, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> h(2 * N);
+            HIPCHK(hipMemcpy(h.data(), src, 8 * N, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < 2 * N; i++) dst[i] = h[i];
+        }
+    }
+    return TDF_OK;
+}
+
+int tdf_debug_fstat(tdf_ctx *c, int iss, double *F)
+{
+    if (!c || !F) return TDF_ERR_ARG;
+    const int slot = iss - c->last_batch_iss0;
+    if (slot < 0 || slot >= c->last_batch_n) return TDF_ERR_ARG;
+    const FftPlan &p = c->p_spin;
+    const size_t n = p.n;
+    std::vector<double> ft(n);
+    const char *src = (const char *)c->FT + slot * n * c->rsz;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->prec == TDF_PREC_FP64) {
+        HIPCHK(hipMemcpy(ft.data(), src, 8 * n, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> h(n);
+        HIPCHK(hipMemcpy(h.data(), src, 4 * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) ft[i] = h[i];
+    }
+    // FT[k2*n1 + k1] = F[k2 + n2*k1]
+    for (size_t k = 0; k < n; k++) F[k] = 0.;
+    for (int k = c->last_fr0; k <= c->last_fr1; k++) F[k] = ft[(size_t)(k % p.n2) * p.n1 + k / p.n2];
+    return TDF_OK;
+}
+
+int tdf_fft_c2c(const double *in, double *out, int n, int howmany, int sign, int prec)
+{
+    if (!in || !out || howmany < 1 || (sign != -1 && sign != 1)) return TDF_ERR_ARG;
+    if (prec != TDF_PREC_FP32 && prec != TDF_PREC_FP64) return TDF_ERR_ARG;
+    tdf_geom g{};
+    g.N = n;
+    g.nfft = n;
+    g.Ninterp = 2 * n;
+    g.nfftf = n;
+    g.interpftpad = 2;
+    g.nifo = 1;
+    tdf_ctx *c = nullptr;
+    int rc = tdf_ctx_create(&g, prec, 1, &c);
+    if (rc) return rc;
+    const int npairs = (howmany + 1) / 2;
+    const size_t cs = c->csz;
+    void *din = nullptr, *dtmp = nullptr, *dT = nullptr, *dout = nullptr;
+    const size_t bytes = cs * 2 * npairs * (size_t)n;
+    rc = dmalloc(c, &din, bytes);
+    if (!rc) rc = dmalloc(c, &dtmp, bytes);
+    if (!rc) rc = dmalloc(c, &dT, bytes);
+    if (!rc) rc = dmalloc(c, &dout, bytes);
+    if (rc) {
+        tdf_ctx_destroy(c);
+        return rc;
+    }
+    const size_t tot = 2 * (size_t)n * howmany;
+    std::vector<char> h(bytes, 0);
+    if (prec == TDF_PREC_FP64) memcpy(h.data(), in, sizeof(double) * tot);
+    else
+        for (size_t i = 0; i < tot; i++) ((float *)h.data())[i] = (float)in[i];
+    hipError_t e = hipMemcpy(din, h.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (prec == TDF_PREC_FP64) {
+            e = launch_fft_natural_to_T<double>(c->p_fwd, din, dtmp, dT, npairs, sign, 0, 0, 1.0, c->stream);
+            if (e == hipSuccess)
+                e = launch_transpose<double>(c->p_fwd, dT, dout, 2 * npairs, n, n, n, 0, n, c->stream);
+        } else {
+            e = launch_fft_natural_to_T<float>(c->p_fwd, din, dtmp, dT, npairs, sign, 0, 0, 1.0, c->stream);
+            if (e == hipSuccess)
+                e = launch_transpose<float>(c->p_fwd, dT, dout, 2 * npairs, n, n, n, 0, n, c->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), dout, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+        if (prec == TDF_PREC_FP64) memcpy(out, h.data(), sizeof(double) * tot);
+        else
+            for (size_t i = 0; i < tot; i++) out[i] = ((float *)h.data())[i];
+    }
+    tdf_ctx_destroy(c);
+    if (e != hipSuccess) {
+        fprintf(stderr, "[tdfstat_hip] fft: %s\n", hipGetErrorString(e));
+        return TDF_ERR_HIP;
+    }
+    return TDF_OK;
+}
+
+int tdf_profile_enable(tdf_ctx *c, int on)
+{
+    if (!c) return TDF_ERR_ARG;
+    c->prof = on != 0;
+    for (int k = 0; k < KS_COUNT; k++) {
+        c->kms[k] = 0.;
+        c->kcount[k] = 0;
+    }
+    c->ev_used = 0;
+    return TDF_OK;
+}
+
+int tdf_kernel_times(tdf_ctx *c, int slot, double *ms, long long *launches, const char **name)
+{
+    if (!c || slot < 0) return TDF_ERR_ARG;
+    if (slot >= KS_COUNT) return TDF_ERR_ARG;
+    if (ms) *ms = c->kms[slot];
+    if (launches) *launches = c->kcount[slot];
+    if (name) *name = kslot_names[slot];
+    return TDF_OK;
+}
+
+int tdf_synchronize(tdf_ctx *c)
+{
+    if (!c) return TDF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+    return TDF_OK;
+}
+
+}  // extern "C"

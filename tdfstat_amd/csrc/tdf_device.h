@@ -1,0 +1,264 @@
+// Device building blocks for the gfx950 TDFstat kernels: complex helpers,
+// register DFTs (radix 2/4/8/16), the LDS-staged Stockham FFT used inside
+// one workgroup, FP64 phase reduction with an FP32 sin/cos, and the natural
+// cubic-spline filter taps.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tdf {
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmk(T a, T b) { return cx<T>{a, b}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__device__ __forceinline__ T cnorm(cx<T> a) { return a.x * a.x + a.y * a.y; }
+
+// a * (SIGN * i):  SIGN=-1 -> a*(-i) = (y, -x);  SIGN=+1 -> a*i = (-y, x)
+template <int SIGN, typename T>
+__device__ __forceinline__ cx<T> mul_si(cx<T> a)
+{
+    return SIGN < 0 ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x};
+}
+
+// ---------------------------------------------------------------------
+// Register DFTs, V[p] = sum_q v[q] exp(SIGN*2*pi*i*q*p/R), natural order.
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft2(cx<T> *v)
+{
+    cx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft4s(cx<T> &v0, cx<T> &v1, cx<T> &v2, cx<T> &v3)
+{
+    cx<T> a0 = v0 + v2, a1 = v0 - v2, a2 = v1 + v3, a3 = mul_si<SIGN>(v1 - v3);
+    v0 = a0 + a2;
+    v2 = a0 - a2;
+    v1 = a1 + a3;
+    v3 = a1 - a3;
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft4(cx<T> *v)
+{
+    dft4s<SIGN>(v[0], v[1], v[2], v[3]);
+}
+
+// R=8 as 4x2: q = q1 + 2*q2, p = p1 + 4*p2
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft8(cx<T> *v)
+{
+    const T h = (T)0.70710678118654752440;
+    cx<T> a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+    cx<T> b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+    dft4s<SIGN>(a0, a1, a2, a3);
+    dft4s<SIGN>(b0, b1, b2, b3);
+    // twiddle W8^{p1}, W8 = exp(SIGN*i*pi/4)
+    b1 = cmul(b1, cx<T>{h, (T)SIGN * h});
+    b2 = mul_si<SIGN>(b2);
+    b3 = cmul(b3, cx<T>{-h, (T)SIGN * h});
+    v[0] = a0 + b0;
+    v[4] = a0 - b0;
+    v[1] = a1 + b1;
+    v[5] = a1 - b1;
+    v[2] = a2 + b2;
+    v[6] = a2 - b2;
+    v[3] = a3 + b3;
+    v[7] = a3 - b3;
+}
+
+// R=16 as 4x4: q = q1 + 4*q2, p = p1 + 4*p2
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft16(cx<T> *v)
+{
+    // cos(k*pi/8), sin(k*pi/8), k = 0..9
+    const T C[10] = {(T)1.0,
+                     (T)0.92387953251128675613,
+                     (T)0.70710678118654752440,
+                     (T)0.38268343236508977173,
+                     (T)0.0,
+                     (T)-0.38268343236508977173,
+                     (T)-0.70710678118654752440,
+                     (T)-0.92387953251128675613,
+                     (T)-1.0,
+                     (T)-0.92387953251128675613};
+    const T S[10] = {(T)0.0,
+                     (T)0.38268343236508977173,
+                     (T)0.70710678118654752440,
+                     (T)0.92387953251128675613,
+                     (T)1.0,
+                     (T)0.92387953251128675613,
+                     (T)0.70710678118654752440,
+                     (T)0.38268343236508977173,
+                     (T)0.0,
+                     (T)-0.38268343236508977173};
+    cx<T> a[4][4];
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) {
+        cx<T> t0 = v[q1], t1 = v[q1 + 4], t2 = v[q1 + 8], t3 = v[q1 + 12];
+        dft4s<SIGN>(t0, t1, t2, t3);
+        a[q1][0] = t0;
+        a[q1][1] = t1;
+        a[q1][2] = t2;
+        a[q1][3] = t3;
+    }
+#pragma unroll
+    for (int q1 = 1; q1 < 4; q1++)
+#pragma unroll
+        for (int p1 = 1; p1 < 4; p1++) {
+            const int k = q1 * p1;
+            a[q1][p1] = cmul(a[q1][p1], cx<T>{C[k], (T)SIGN * S[k]});
+        }
+#pragma unroll
+    for (int p1 = 0; p1 < 4; p1++) {
+        cx<T> t0 = a[0][p1], t1 = a[1][p1], t2 = a[2][p1], t3 = a[3][p1];
+        dft4s<SIGN>(t0, t1, t2, t3);
+        v[p1] = t0;
+        v[p1 + 4] = t1;
+        v[p1 + 8] = t2;
+        v[p1 + 12] = t3;
+    }
+}
+
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dftR(cx<T> *v)
+{
+    if constexpr (R == 2) dft2<SIGN>(v);
+    else if constexpr (R == 4) dft4<SIGN>(v);
+    else if constexpr (R == 8) dft8<SIGN>(v);
+    else dft16<SIGN>(v);
+}
+
+// ---------------------------------------------------------------------
+// LDS tile of NCOL columns, each L complex values; column stride LP with one
+// pad slot per 16 elements to break the power-of-two strides of the
+// Stockham writes.
+template <int L>
+struct ldsgeom {
+    static constexpr int LP = L + L / 16;
+    __device__ static __forceinline__ int pad(int p) { return p + (p >> 4); }
+};
+
+// One Stockham radix-RS stage (NS = product of the earlier radices) over all
+// NCOL columns.  Each of the NT = NCOL*L/16 threads owns 16/RS butterflies.
+// twL[m] = exp(-2 pi i m / L).
+template <typename T, int SIGN, int L, int NCOL, int RS, int NS>
+__device__ __forceinline__ void stockham_stage(cx<T> *tile, const cx<T> *__restrict__ twL)
+{
+    constexpr int NT = NCOL * L / 16;
+    constexpr int NB = 16 / RS;
+    constexpr int LR = L / RS;
+    constexpr int LP = ldsgeom<L>::LP;
+    const int tid = threadIdx.x;
+    cx<T> v[16];
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+        const int g = tid + u * NT;
+        const int c = g / LR, j = g % LR;
+#pragma unroll
+        for (int r = 0; r < RS; r++) v[u * RS + r] = tile[c * LP + ldsgeom<L>::pad(j + r * LR)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+        const int g = tid + u * NT;
+        const int c = g / LR, j = g % LR;
+        const int k = j % NS;
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < RS; r++) {
+                cx<T> w = twL[r * k * (L / (NS * RS))];
+                if constexpr (SIGN > 0) w.y = -w.y;
+                v[u * RS + r] = cmul(v[u * RS + r], w);
+            }
+        }
+        dftR<RS, SIGN>(&v[u * RS]);
+        const int idxD = (j / NS) * NS * RS + k;
+#pragma unroll
+        for (int r = 0; r < RS; r++) tile[c * LP + ldsgeom<L>::pad(idxD + r * NS)] = v[u * RS + r];
+    }
+    __syncthreads();
+}
+
+// Full in-LDS FFT of every column (natural order in, natural order out).
+// L = 16^a * {1,2,4,8}: radix-16 stages followed by one smaller stage.
+template <typename T, int SIGN, int L, int NCOL>
+__device__ __forceinline__ void lds_fft(cx<T> *tile, const cx<T> *__restrict__ twL)
+{
+    static_assert(L >= 64 && L <= 4096 && (L & (L - 1)) == 0, "L");
+    stockham_stage<T, SIGN, L, NCOL, 16, 1>(tile, twL);
+    if constexpr (L == 64) {
+        stockham_stage<T, SIGN, L, NCOL, 4, 16>(tile, twL);
+    } else if constexpr (L == 128) {
+        stockham_stage<T, SIGN, L, NCOL, 8, 16>(tile, twL);
+    } else {
+        stockham_stage<T, SIGN, L, NCOL, 16, 16>(tile, twL);
+        if constexpr (L == 512) stockham_stage<T, SIGN, L, NCOL, 2, 256>(tile, twL);
+        if constexpr (L == 1024) stockham_stage<T, SIGN, L, NCOL, 4, 256>(tile, twL);
+        if constexpr (L == 2048) stockham_stage<T, SIGN, L, NCOL, 8, 256>(tile, twL);
+        if constexpr (L == 4096) stockham_stage<T, SIGN, L, NCOL, 16, 256>(tile, twL);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Phase -> exp(-i*phase).  The phase is FP64; for T=float it is reduced
+// modulo pi/2 in FP64 (Cody-Waite, pi/2 split into two doubles) and the
+// remainder |r| <= pi/4 is evaluated with Taylor polynomials in FP32
+// (truncation < 3e-9 relative, below FP32 rounding).
+__device__ __forceinline__ void sincos_fp64arg(double ph, float *s, float *c)
+{
+    const double TWO_OVER_PI = 0.63661977236758134308;
+    const double PIO2_HI = 1.5707963267948965580e+00;
+    const double PIO2_LO = 6.1232339957367658e-17;
+    double kq = rint(ph * TWO_OVER_PI);
+    double r = fma(-kq, PIO2_HI, ph);
+    r = fma(-kq, PIO2_LO, r);
+    const int q = (int)((long long)kq & 3);
+    const float x = (float)r, x2 = x * x;
+    // sin x = x(1 - x^2/3! + x^4/5! - x^6/7! + x^8/9!)
+    float sp = fmaf(x2, 2.7557319224e-6f, -1.9841269841e-4f);
+    sp = fmaf(x2, sp, 8.3333333333e-3f);
+    sp = fmaf(x2, sp, -1.6666666667e-1f);
+    sp = fmaf(x2 * x, sp, x);
+    // cos x = 1 - x^2/2! + x^4/4! - x^6/6! + x^8/8! - x^10/10!
+    float cp = fmaf(x2, -2.7557319224e-7f, 2.4801587302e-5f);
+    cp = fmaf(x2, cp, -1.3888888889e-3f);
+    cp = fmaf(x2, cp, 4.1666666667e-2f);
+    cp = fmaf(x2, cp, -0.5f);
+    cp = fmaf(x2, cp, 1.0f);
+    float so, co;
+    switch (q) {
+    case 0: so = sp; co = cp; break;
+    case 1: so = cp; co = -sp; break;
+    case 2: so = -sp; co = -cp; break;
+    default: so = -cp; co = sp; break;
+    }
+    *s = so;
+    *c = co;
+}
+
+__device__ __forceinline__ void sincos_fp64arg(double ph, double *s, double *c)
+{
+    sincos(ph, s, c);
+}
+
+}  // namespace tdf

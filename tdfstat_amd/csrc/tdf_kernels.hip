@@ -1,0 +1,780 @@
+// gfx950 kernels of the TDFstat coherent-search hot path.
+//
+// Per-sky stage (search/network/openmp/jobcore.c:273-391):
+//   k_sky_prep      modvir (settings.c:254-294) + barycentric shift +
+//                   FP64-phase demodulation (jobcore.c:283-312) + aa/bb
+//                   partial sums (jobcore.c:373-391)
+//   k_fft_pass      two-pass (four-step) Stockham FFT, nfft forward and
+//                   Ninterp inverse (jobcore.c:333-361)
+//   k_transpose     column-major FFT output -> natural order, fused with
+//                   the spectrum move (jobcore.c:340-350)
+//   k_spline_coef   natural cubic-spline second derivatives
+//                   (auxi.c:93-117) as a two-sided IIR with boundary terms
+//   k_splint        spline evaluation at 2(i - shftf[i]) and /sig2
+//                   (auxi.c:120-169, jobcore.c:387-390)
+// Spindown stage (jobcore.c:429-603):
+//   k_fft_pass<IO_SpinP1>  spindown demodulation summed over detectors
+//                          (spinmod.c:490-534) fused into the FFT load
+//   k_fft_pass<IO_SpinP2>  F = |Fa|^2/aa + |Fb|^2/bb fused into the FFT
+//                          store (jobcore.c:480-484)
+//   k_fstat_blocks  FStat block means (auxi.c:336-363)
+//   k_triggers      MAX_ALG 3 local maxima, veto lines, wavefront-ballot
+//                   compaction (jobcore.c:553-603)
+#include "tdf_internal.h"
+
+namespace tdf {
+
+// ---------------------------------------------------------------------
+// Per-sky prep: one thread per sample i < nfft of detector blockIdx.y.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restrict__ xab)
+{
+    const int d = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const DetParams &dp = a.det[d];
+    double a2 = 0., b2 = 0.;
+    if (i < a.nfft) {
+        cx<T> xa = {(T)0, (T)0}, xb = {(T)0, (T)0};
+        if (i < a.N) {
+            const float x = a.xDat[(size_t)d * a.N + i];
+            const double *ssb = a.DetSSB + (size_t)d * 3 * a.N + 3 * (size_t)i;
+            const double shft = a.ns0 * ssb[0] + a.ns1 * ssb[1] + a.ns2 * ssb[2];
+            const double shftf = shft - a.shft1[d];
+            const double tmp1 = (double)i * (double)i + (double)(2 * i) * shft;
+            a.shftf[(size_t)d * a.N + i] = shftf;
+            a.tmp1[(size_t)d * a.N + i] = tmp1;
+            double aa = 0., bb = 0.;
+            if (fabs(x) > 2.2250738585072014e-308) {
+                double sm, cm;
+                sincos(a.omr * i, &sm, &cm);
+                const double c = a.cosalfr[d] * cm + a.sinalfr[d] * sm;
+                const double s = a.sinalfr[d] * cm - a.cosalfr[d] * sm;
+                const double c2s = 2. * c * c;
+                const double cs = c * s;
+                const double *k = dp.c;
+                aa = k[0] * (2. - a.c2d) * c2s + k[1] * (2. - a.c2d) * 2. * cs +
+                     k[2] * a.c2sd * c + k[3] * a.c2sd * s - k[0] * (2. - a.c2d) + k[4] * a.c2d;
+                bb = k[5] * a.sindel * c2s + k[6] * a.sindel * 2. * cs + k[7] * a.cosdel * c +
+                     k[8] * a.cosdel * s - k[5] * a.sindel;
+            }
+            a2 = aa * aa;
+            b2 = bb * bb;
+            const double phase = a.het0 * i + a.oms * shft;
+            T sp, cp;
+            sincos_fp64arg(phase, &sp, &cp);
+            const T xaa = (T)((double)x * aa), xbb = (T)((double)x * bb);
+            xa = cx<T>{xaa * cp, -xaa * sp};
+            xb = cx<T>{xbb * cp, -xbb * sp};
+        }
+        xab[(size_t)(2 * d) * a.nfft + i] = xa;
+        xab[(size_t)(2 * d + 1) * a.nfft + i] = xb;
+    }
+    // block reduction of aa^2, bb^2 (fixed order)
+    __shared__ double ra[256], rb[256];
+    ra[threadIdx.x] = a2;
+    rb[threadIdx.x] = b2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            ra[threadIdx.x] += ra[threadIdx.x + s];
+            rb[threadIdx.x] += rb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.partials[((size_t)d * 2 + 0) * gridDim.x + blockIdx.x] = ra[0];
+        a.partials[((size_t)d * 2 + 1) * gridDim.x + blockIdx.x] = rb[0];
+    }
+}
+
+// aa = sum_d (sum_i aa_d^2)/sig2_d, likewise bb (jobcore.c:373-391)
+__global__ void __launch_bounds__(256) k_sky_reduce(const double *__restrict__ partials, int nblk,
+                                                    int nifo, const double *__restrict__ sig2,
+                                                    double *__restrict__ out)
+{
+    __shared__ double r[2][256];
+    double acc[2] = {0., 0.};
+    for (int d = 0; d < nifo; d++) {
+        for (int w = 0; w < 2; w++) {
+            double s = 0.;
+            for (int b = threadIdx.x; b < nblk; b += 256) s += partials[((size_t)d * 2 + w) * nblk + b];
+            r[w][threadIdx.x] = s;
+        }
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                r[0][threadIdx.x] += r[0][threadIdx.x + s];
+                r[1][threadIdx.x] += r[1][threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            acc[0] += r[0][0] / sig2[d];
+            acc[1] += r[1][0] / sig2[d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = acc[0];
+        out[1] = acc[1];
+    }
+}
+
+// ---------------------------------------------------------------------
+// Generic FFT pass: cooperative load into the LDS tile, in-LDS Stockham
+// FFT of every column, cooperative store.  IO supplies load/store.
+template <typename T, int SIGN, int L, class IO>
+__global__ void __launch_bounds__(FftCfg<T>::NT) k_fft_pass(IO io, const cx<T> *__restrict__ twL)
+{
+    constexpr int NCOL = FftCfg<T>::ELEMS / L;
+    __shared__ cx<T> tile[NCOL * ldsgeom<L>::LP];
+    io.template load<SIGN, L, NCOL>(tile);
+    __syncthreads();
+    lds_fft<T, SIGN, L, NCOL>(tile, twL);
+    io.template store<SIGN, L, NCOL>(tile);
+}
+
+// W_n^e from the two-level table (e < n), conjugated for SIGN>0
+template <int SIGN, typename T>
+__device__ __forceinline__ cx<T> twiddle_n(const cx<T> *__restrict__ lo, const cx<T> *__restrict__ hi,
+                                           unsigned e)
+{
+    cx<T> w = cmul(lo[e & (TW_LO - 1)], hi[e >> TW_LO_LOG2]);
+    if constexpr (SIGN > 0) w.y = -w.y;
+    return w;
+}
+
+// Pass 1 on natural-order arrays (pairs 2*item, 2*item+1 of stride n):
+// columns c in [c0, c0+TC), element q of column c is x[c + n1*q].
+// Indices in [gap0, gap1) read as zero (spectrum-move gap).
+template <typename T>
+struct IO_P1Nat {
+    const cx<T> *in;
+    cx<T> *out;
+    const cx<T> *tw_lo, *tw_hi;
+    int n, n1, gap0, gap1;
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void load(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const size_t base = (size_t)(2 * blockIdx.y) * n;
+        const int c0 = blockIdx.x * TC;
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int cl = e % TC, q = e / TC;
+            const int i = c0 + cl + n1 * q;
+            cx<T> va = {(T)0, (T)0}, vb = {(T)0, (T)0};
+            if (i < gap0 || i >= gap1) {
+                va = in[base + i];
+                vb = in[base + n + i];
+            }
+            tile[cl * LP + ldsgeom<L>::pad(q)] = va;
+            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = vb;
+        }
+    }
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void store(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const size_t base = (size_t)(2 * blockIdx.y) * n;
+        const int c0 = blockIdx.x * TC;
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int cl = e % TC, k2 = e / TC;
+            const int c = c0 + cl;
+            const cx<T> w = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * k2));
+            out[base + (size_t)k2 * n1 + c] = cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w);
+            out[base + n + (size_t)k2 * n1 + c] = cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w);
+        }
+    }
+};
+
+// Pass 2 writing the column-major ("transposed") result:
+// out[k2*n1 + k1] = X[k2 + n2*k1] * scale.
+template <typename T>
+struct IO_P2T {
+    const cx<T> *in;
+    cx<T> *out;
+    int n, n1;
+    T scale;
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void load(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const size_t base = (size_t)(2 * blockIdx.y) * n;
+        const int k20 = blockIdx.x * TC;
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int q = e % L, cl = e / L;
+            const size_t off = (size_t)(k20 + cl) * n1 + q;
+            tile[cl * LP + ldsgeom<L>::pad(q)] = in[base + off];
+            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = in[base + n + off];
+        }
+    }
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void store(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const size_t base = (size_t)(2 * blockIdx.y) * n;
+        const int k20 = blockIdx.x * TC;
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int k1 = e % L, cl = e / L;
+            const size_t off = (size_t)(k20 + cl) * n1 + k1;
+            out[base + off] = cscale(tile[cl * LP + ldsgeom<L>::pad(k1)], scale);
+            out[base + n + off] = cscale(tile[(TC + cl) * LP + ldsgeom<L>::pad(k1)], scale);
+        }
+    }
+};
+
+// Spindown pass 1: load = demodulation summed over detectors
+// (spinmod.c:490-514 semantics with an FP64 phase), store = column FFT
+// times W_nfftf^(c*k2) into Z[slot][a|b][k2*n1 + c].
+template <typename T>
+struct IO_SpinP1 {
+    const cx<T> *xm;       // [nifo][2][N]
+    const double *tmp1;    // [nifo][N]
+    const TmplDesc *desc;  // [batch]
+    cx<T> *Z;              // [batch][2][n]
+    const cx<T> *tw_lo, *tw_hi;
+    int N, n, n1, nifo, slot0;
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void load(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const TmplDesc td = desc[slot0 + blockIdx.y];
+        const int c0 = blockIdx.x * TC;
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int cl = e % TC, q = e / TC;
+            const int i = c0 + cl + n1 * q;
+            cx<T> fa = {(T)0, (T)0}, fb = {(T)0, (T)0};
+            if (i < N) {
+                const double ph0 = td.het1 * (double)i;
+                for (int d = 0; d < nifo; d++) {
+                    const double ph = ph0 + td.spnd * tmp1[(size_t)d * N + i];
+                    T s, c;
+                    sincos_fp64arg(ph, &s, &c);
+                    const cx<T> ma = xm[(size_t)(2 * d) * N + i];
+                    const cx<T> mb = xm[(size_t)(2 * d + 1) * N + i];
+                    fa.x += ma.x * c + ma.y * s;
+                    fa.y += ma.y * c - ma.x * s;
+                    fb.x += mb.x * c + mb.y * s;
+                    fb.y += mb.y * c - mb.x * s;
+                }
+            }
+            tile[cl * LP + ldsgeom<L>::pad(q)] = fa;
+            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = fb;
+        }
+    }
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void store(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const size_t base = (size_t)blockIdx.y * 2 * n;
+        const int c0 = blockIdx.x * TC;
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int cl = e % TC, k2 = e / TC;
+            const int c = c0 + cl;
+            const cx<T> w = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * k2));
+            Z[base + (size_t)k2 * n1 + c] = cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w);
+            Z[base + n + (size_t)k2 * n1 + c] = cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w);
+        }
+    }
+};
+
+// Spindown pass 2: store = F-statistic of bins fr0..fr1 into
+// FT[slot][k2*n1 + k1] (bin k = k2 + n2*k1).
+template <typename T>
+struct IO_SpinP2 {
+    const cx<T> *Z;
+    T *FT;
+    const double *sky;  // aa, bb
+    int n, n1, n2, fr0, fr1;
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void load(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const size_t base = (size_t)blockIdx.y * 2 * n;
+        const int k20 = blockIdx.x * TC;
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int q = e % L, cl = e / L;
+            const size_t off = (size_t)(k20 + cl) * n1 + q;
+            tile[cl * LP + ldsgeom<L>::pad(q)] = Z[base + off];
+            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = Z[base + n + off];
+        }
+    }
+    template <int SIGN, int L, int NCOL>
+    __device__ __forceinline__ void store(cx<T> *tile)
+    {
+        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        const size_t base = (size_t)blockIdx.y * n;
+        const int k20 = blockIdx.x * TC;
+        const T aa = (T)sky[0], bb = (T)sky[1];
+        for (int e = threadIdx.x; e < TC * L; e += NT) {
+            const int k1 = e % L, cl = e / L;
+            const int k2 = k20 + cl;
+            const int k = k2 + n2 * k1;
+            if (k >= fr0 && k <= fr1) {
+                const T F = cnorm(tile[cl * LP + ldsgeom<L>::pad(k1)]) / aa +
+                            cnorm(tile[(TC + cl) * LP + ldsgeom<L>::pad(k1)]) / bb;
+                FT[base + (size_t)k2 * n1 + k1] = F;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------
+// Batched transpose of column-major FFT output: in[item][k2*n1 + k1]
+// (rows k2 < n2, cols k1 < n1) -> natural bin k = k2 + n2*k1, written to
+// out[item][dst(k)], dst(k) = k for k < split, k + shift otherwise; only
+// k < kmax are written.
+template <typename T>
+__global__ void __launch_bounds__(256) k_transpose(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
+                                                   int n1, int n2, size_t in_stride, size_t out_stride,
+                                                   int split, int shift, int kmax)
+{
+    __shared__ cx<T> t[32][33];
+    const int item = blockIdx.z;
+    const int k1b = blockIdx.x * 32, k2b = blockIdx.y * 32;
+    const cx<T> *src = in + (size_t)item * in_stride;
+    cx<T> *dst = out + (size_t)item * out_stride;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int k2 = k2b + r, k1 = k1b + tx;
+        if (k2 < n2 && k1 < n1) t[r][tx] = src[(size_t)k2 * n1 + k1];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int k1 = k1b + r, k2 = k2b + tx;
+        if (k2 < n2 && k1 < n1) {
+            const int k = k2 + n2 * k1;
+            if (k < kmax) dst[k < split ? k : k + shift] = t[tx][r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Natural cubic spline second derivatives on the unit grid
+// (auxi.c:93-117).  With d_j = 6(y_{j-1} - 2y_j + y_{j+1}) for
+// 1 <= j <= n-2 (0 elsewhere) and r = 2 - sqrt(3), the tridiagonal system
+// s_{i-1} + 4 s_i + s_{i+1} = d_i, s_0 = s_{n-1} = 0 has the solution
+//   p_i = 1/(2 sqrt 3) * sum_k (-r)^|k| d_{i+k}
+//   s_i = p_i - p_0 (-r)^i - p_{n-1} (-r)^(n-1-i)
+// (exact up to (-r)^(n-1) and the tap truncation (-r)^(K+1)).
+template <typename T>
+__global__ void __launch_bounds__(256) k_spline_coef(const cx<T> *__restrict__ Y, cx<T> *__restrict__ Y2,
+                                                     int n, size_t ystride, size_t y2stride,
+                                                     const double *__restrict__ taps)
+{
+    constexpr int K = SplineCfg<T>::K;
+    constexpr int W = 256 + 2 * K + 2;  // y window
+    __shared__ cx<T> ys[W];
+    __shared__ cx<T> ds[256 + 2 * K];
+    __shared__ cx<T> pb[2];
+    const int item = blockIdx.y;
+    const int i0 = blockIdx.x * 256;
+    const cx<T> *y = Y + (size_t)item * ystride;
+    for (int e = threadIdx.x; e < W; e += 256) {
+        const int j = i0 - K - 1 + e;
+        ys[e] = (j >= 0 && j < n) ? y[j] : cx<T>{(T)0, (T)0};
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256 + 2 * K; e += 256) {
+        const int j = i0 - K + e;
+        cx<T> d = {(T)0, (T)0};
+        if (j >= 1 && j <= n - 2) {
+            const cx<T> a = ys[e], b = ys[e + 1], c = ys[e + 2];
+            d = cscale((a - cscale(b, (T)2)) + c, (T)6);
+        }
+        ds[e] = d;
+    }
+    // boundary particular values p_0, p_{n-1} (only blocks near an edge use them)
+    const bool nearlo = i0 < SplineCfg<T>::BK;
+    const bool nearhi = i0 + 256 > n - 1 - SplineCfg<T>::BK;
+    if (threadIdx.x < 2) {
+        cx<T> acc = {(T)0, (T)0};
+        const bool want = threadIdx.x == 0 ? nearlo : nearhi;
+        if (want) {
+            for (int k = K; k >= 1; k--) {
+                const int j = threadIdx.x == 0 ? k : n - 1 - k;
+                if (j >= 1 && j <= n - 2) {
+                    const cx<T> d = cscale((y[j - 1] - cscale(y[j], (T)2)) + y[j + 1], (T)6);
+                    acc = acc + cscale(d, (T)taps[k]);
+                }
+            }
+        }
+        pb[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.x;
+    if (i < n) {
+        const int e = threadIdx.x + K;
+        cx<T> acc = cscale(ds[e], (T)taps[0]);
+#pragma unroll
+        for (int k = 1; k <= K; k++) acc = acc + cscale(ds[e - k] + ds[e + k], (T)taps[k]);
+        if (nearlo && i < SplineCfg<T>::BK) acc = acc - cscale(pb[0], (T)taps[K + 1 + i]);
+        const int ih = n - 1 - i;
+        if (nearhi && ih < SplineCfg<T>::BK) acc = acc - cscale(pb[1], (T)taps[K + 1 + ih]);
+        Y2[(size_t)item * y2stride + i] = acc;
+    }
+}
+
+// splint (auxi.c:120-133) at x = interpftpad*(i - shftf[i]) for i < N,
+// divided by sig2 (jobcore.c:387-390).  Items are (det, a|b) pairs.
+template <typename T>
+__global__ void __launch_bounds__(256) k_splint(const cx<T> *__restrict__ Y, const cx<T> *__restrict__ Y2,
+                                                const double *__restrict__ shftf,
+                                                const double *__restrict__ sig2, cx<T> *__restrict__ xm,
+                                                int N, int n, int ipad, size_t ystride,
+                                                size_t y2stride)
+{
+    const int item = blockIdx.y, d = item >> 1;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const double x = ipad * (i - shftf[(size_t)d * N + i]);
+    cx<T> out = {(T)0, (T)0};
+    if (!(x < 0 || x > n - 1)) {
+        const int klo = (int)floor(x);
+        const int khi = klo + 1;
+        const T a = (T)(khi - x), b = (T)(x - klo);
+        const cx<T> *y = Y + (size_t)item * ystride;
+        const cx<T> *y2 = Y2 + (size_t)item * y2stride;
+        // khi == n only for x == n-1 exactly, where b == 0
+        const cx<T> yhi = khi < n ? y[khi] : cx<T>{(T)0, (T)0};
+        const cx<T> y2hi = khi < n ? y2[khi] : cx<T>{(T)0, (T)0};
+        const cx<T> lin = cscale(y[klo], a) + cscale(yhi, b);
+        const cx<T> cur = cscale(y2[klo], a * a * a - a) + cscale(y2hi, b * b * b - b);
+        out = lin + cx<T>{cur.x / (T)6.0, cur.y / (T)6.0};
+    }
+    const T s2 = (T)sig2[d];
+    xm[(size_t)item * N + i] = cx<T>{out.x / s2, out.y / s2};
+}
+
+// ---------------------------------------------------------------------
+// F-statistic tiles: natural bins [s, s+TB) of one spindown slot, read
+// from the column-major FT with lanes along k1 (contiguous in FT).
+template <typename T>
+__device__ __forceinline__ void load_f_tile(const T *__restrict__ FT, T *nat, int s, int lo, int hi,
+                                            int fr0, int fr1, int n1, int n2)
+{
+    // nat[k - lo] = F[k] for k in [lo, hi) (0 outside [fr0, fr1])
+    const int k1a = (lo < 0 ? 0 : lo) / n2;
+    const int k1b = (hi - 1) / n2;
+    const int dk = k1b - k1a + 1;
+    const int tot = dk * n2;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int k1 = k1a + e % dk, k2 = e / dk;
+        const int k = k2 + n2 * k1;
+        if (k >= lo && k < hi) nat[k - lo] = (k >= fr0 && k <= fr1) ? FT[(size_t)k2 * n1 + k1] : (T)0;
+    }
+}
+
+// Block sums for FStat: tile t covers [fr0 + t*TB, fr0 + (t+1)*TB) which
+// holds whole blocks (TB is a multiple of nav).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fstat_blocks(const T *__restrict__ FT, T *__restrict__ mu,
+                                                      FstatArgs a)
+{
+    constexpr int TRIG_TB = TrigCfg<T>::TB;
+    __shared__ T nat[TRIG_TB];
+    __shared__ T cs[256];
+    const int slot = blockIdx.y;
+    const T *ft = FT + (size_t)slot * a.n;
+    const int s = a.fr0 + blockIdx.x * TRIG_TB;
+    const int end_norm = a.fr0 + a.nblk * a.nav;
+    const int hi = min(s + TRIG_TB, end_norm);
+    if (s >= hi) return;
+    load_f_tile(ft, nat, s, s, hi, a.fr0, a.fr1, a.n1, a.n2);
+    __syncthreads();
+    constexpr int CH = TRIG_TB / 256;
+    const int nvalid = hi - s;
+    if (a.nav >= CH) {
+        // chunk of CH consecutive bins per thread, then tree within blocks
+        T acc = 0;
+        const int b0 = threadIdx.x * CH;
+        for (int u = 0; u < CH; u++) {
+            const int b = b0 + u;
+            if (b < nvalid) acc += nat[b];
+        }
+        cs[threadIdx.x] = acc;
+        __syncthreads();
+        const int cpb = a.nav / CH;  // chunks per block
+        for (int st = 1; st < cpb; st <<= 1) {
+            if ((threadIdx.x % (2 * st)) == 0) cs[threadIdx.x] += cs[threadIdx.x + st];
+            __syncthreads();
+        }
+        if ((threadIdx.x % cpb) == 0 && threadIdx.x * CH < nvalid) {
+            const int blk = (s - a.fr0) / a.nav + threadIdx.x / cpb;
+            mu[(size_t)slot * a.nblk_cap + blk] = (T)((double)cs[threadIdx.x] / (2. * a.nav));
+        }
+    } else {
+        // nav < CH: each thread sums whole blocks sequentially
+        const int nb = nvalid / a.nav;
+        for (int bl = threadIdx.x; bl < nb; bl += 256) {
+            T acc = 0;
+            for (int u = 0; u < a.nav; u++) acc += nat[bl * a.nav + u];
+            const int blk = (s - a.fr0) / a.nav + bl;
+            mu[(size_t)slot * a.nblk_cap + blk] = (T)((double)acc / (2. * a.nav));
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T fnorm_at(const T *__restrict__ ft, const T *__restrict__ mu, const TrigArgs &a,
+                                      int k)
+{
+    const int k2 = k % a.n2, k1 = k / a.n2;
+    T f = ft[(size_t)k2 * a.n1 + k1];
+    if (a.fstat_norm && k < a.fr0 + a.nblk * a.nav) f = f / mu[(k - a.fr0) / a.nav];
+    return f;
+}
+
+template <typename T>
+__device__ __forceinline__ bool cond_global(const T *__restrict__ ft, const T *__restrict__ mu,
+                                            const TrigArgs &a, int i)
+{
+    const T fi = fnorm_at(ft, mu, a, i);
+    return !(fi < (T)a.thr || fnorm_at(ft, mu, a, i - 1) > fi || fi < fnorm_at(ft, mu, a, i + 1));
+}
+
+// MAX_ALG 3 (jobcore.c:556-603) with the sequential skip rule: i is
+// accepted iff cond(i) holds and the number of consecutive cond-true
+// indices immediately before i (down to fr0+1) is even.
+template <typename T>
+__global__ void __launch_bounds__(256) k_triggers(const T *__restrict__ FT, const T *__restrict__ MU,
+                                                  TrigArgs a, TrigEntry *__restrict__ out,
+                                                  int *__restrict__ counters)
+{
+    constexpr int TRIG_TB = TrigCfg<T>::TB;
+    __shared__ T nat[TRIG_TB + 2];
+    __shared__ unsigned char cf[TRIG_TB];
+    const int slot = blockIdx.y;
+    const T *ft = FT + (size_t)slot * a.n;
+    const T *mu = MU + (size_t)slot * a.nblk_cap;
+    const int s = a.fr0 + blockIdx.x * TRIG_TB;
+    const int e_end = min(s + TRIG_TB, a.fr1 + 1);  // bins of this tile
+    if (s >= e_end) return;
+    load_f_tile(ft, nat, s, s - 1, e_end + 1, a.fr0, a.fr1, a.n1, a.n2);
+    __syncthreads();
+    const int end_norm = a.fr0 + a.nblk * a.nav;
+    if (a.fstat_norm) {
+        for (int e = threadIdx.x; e < e_end + 1 - (s - 1); e += 256) {
+            const int k = s - 1 + e;
+            if (k >= a.fr0 && k < end_norm) nat[e] = nat[e] / mu[(k - a.fr0) / a.nav];
+        }
+        __syncthreads();
+    }
+    const int ilo = a.fr0 + 1, ihi = a.fr1 - 2;  // candidate range (jobcore.c:557)
+    for (int e = threadIdx.x; e < TRIG_TB; e += 256) {
+        const int i = s + e;
+        bool c = false;
+        if (i >= ilo && i <= ihi && i < e_end) {
+            const T fi = nat[e + 1];
+            c = !(fi < (T)a.thr || nat[e] > fi || fi < nat[e + 2]);
+        }
+        cf[e] = c;
+    }
+    __syncthreads();
+    const double sgnl0 = a.desc[slot].sgnl0;
+    for (int e0 = 0; e0 < TRIG_TB; e0 += 256) {
+        const int e = e0 + threadIdx.x;
+        const int i = s + e;
+        bool emit = false;
+        float fo = 0.f, Fo = 0.f;
+        if (cf[e]) {
+            // run parity
+            int run = 0;
+            int j = e - 1;
+            while (j >= 0 && cf[j]) {
+                run++;
+                j--;
+            }
+            if (j < 0) {
+                int ig = s - 1;
+                while (ig >= ilo && cond_global(ft, mu, a, ig)) {
+                    run++;
+                    ig--;
+                }
+            }
+            if ((run & 1) == 0) {
+                const double f_ = (double)(2 * i) / (double)a.nfftf * 3.14159265358979323846 + sgnl0;
+                bool veto = false;
+                for (int l = 0; l < a.numlines; l++)
+                    if (f_ >= a.lines[2 * l] && f_ <= a.lines[2 * l + 1]) {
+                        veto = true;
+                        break;
+                    }
+                if (!veto) {
+                    emit = true;
+                    fo = (float)f_;
+                    Fo = (float)nat[e + 1];
+                }
+            }
+        }
+        // wavefront ballot compaction: one atomic per wave
+        const unsigned long long m = __ballot(emit);
+        if (m) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == leader) {
+                base = atomicAdd(&counters[0], __popcll(m));
+                atomicAdd(&counters[1 + slot], __popcll(m));
+            }
+            base = __shfl(base, leader, 64);
+            if (emit) {
+                const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+                if (pos < a.cap) out[pos] = TrigEntry{slot, i, fo, Fo};
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// launchers
+
+template <typename T, int SIGN, class IO>
+static hipError_t launch_pass(int L, IO io, dim3 grid, const cx<T> *twL, hipStream_t st)
+{
+    switch (L) {
+#define TDF_CASE(LL)                                                                                   \
+    case LL:                                                                                           \
+        if constexpr (FftCfg<T>::ELEMS / LL >= 2) {                                                    \
+            hipLaunchKernelGGL((k_fft_pass<T, SIGN, LL, IO>), grid, dim3(FftCfg<T>::NT), 0, st, io,  \
+                               twL);                                                                   \
+            return hipGetLastError();                                                                  \
+        }                                                                                              \
+        break;
+        TDF_CASE(64)
+        TDF_CASE(128)
+        TDF_CASE(256)
+        TDF_CASE(512)
+        TDF_CASE(1024)
+        TDF_CASE(2048)
+#undef TDF_CASE
+    default:
+        break;
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_sky_prep(const SkyPrepArgs &a, void *xab, int nblk, int nifo, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sky_prep<T>, dim3(nblk, nifo), dim3(256), 0, st, a, (cx<T> *)xab);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_reduce(const double *partials, int nblk, int nifo, const double *sig2, double *out,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sky_reduce, dim3(1), dim3(256), 0, st, partials, nblk, nifo, sig2, out);
+    return hipGetLastError();
+}
+
+// two-pass FFT on nitems pairs of natural-order arrays; result column-major
+template <typename T>
+hipError_t launch_fft_natural_to_T(const FftPlan &p, const void *in, void *tmp, void *outT, int npairs, int sign,
+                                   int gap0, int gap1, double scale, hipStream_t st)
+{
+    const cx<T> *twL1 = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2]; // pass-1 length n2
+    const cx<T> *twL2 = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
+    IO_P1Nat<T> io1{(const cx<T> *)in, (cx<T> *)tmp, (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D],
+                    (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D], p.n, p.n1, gap0, gap1};
+    const int tc1 = FftCfg<T>::ELEMS / p.n2 / 2;
+    const int tc2 = FftCfg<T>::ELEMS / p.n1 / 2;
+    hipError_t e;
+    if (sign < 0) e = launch_pass<T, -1>(p.n2, io1, dim3(p.n1 / tc1, npairs), twL1, st);
+    else e = launch_pass<T, +1>(p.n2, io1, dim3(p.n1 / tc1, npairs), twL1, st);
+    if (e != hipSuccess) return e;
+    IO_P2T<T> io2{(const cx<T> *)tmp, (cx<T> *)outT, p.n, p.n1, (T)scale};
+    if (sign < 0) e = launch_pass<T, -1>(p.n1, io2, dim3(p.n2 / tc2, npairs), twL2, st);
+    else e = launch_pass<T, +1>(p.n1, io2, dim3(p.n2 / tc2, npairs), twL2, st);
+    return e;
+}
+
+template <typename T>
+hipError_t launch_transpose(const FftPlan &p, const void *inT, void *out, int nitems, size_t in_stride,
+                            size_t out_stride, int split, int shift, int kmax, hipStream_t st)
+{
+    dim3 grid((p.n1 + 31) / 32, (p.n2 + 31) / 32, nitems);
+    hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), 0, st, (const cx<T> *)inT, (cx<T> *)out, p.n1, p.n2,
+                       in_stride, out_stride, split, shift, kmax);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_spline(const void *Y, void *Y2, int n, size_t ystride, size_t y2stride, int nitems,
+                         const double *taps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_spline_coef<T>, dim3((n + 255) / 256, nitems), dim3(256), 0, st, (const cx<T> *)Y,
+                       (cx<T> *)Y2, n, ystride, y2stride, taps);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, const double *sig2, void *xm, int N,
+                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_splint<T>, dim3((N + 255) / 256, nitems), dim3(256), 0, st, (const cx<T> *)Y,
+                       (const cx<T> *)Y2, shftf, sig2, (cx<T> *)xm, N, n, ipad, ystride, y2stride);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st)
+{
+    IO_SpinP1<T> io{(const cx<T> *)s.xm, s.tmp1, s.desc, (cx<T> *)s.Z, (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D],
+                    (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D], s.N, p.n, p.n1, s.nifo, s.slot0};
+    const int tc1 = FftCfg<T>::ELEMS / p.n2 / 2;
+    return launch_pass<T, -1>(p.n2, io, dim3(p.n1 / tc1, s.nslots),
+                              (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2], st);
+}
+
+template <typename T>
+hipError_t launch_spin_pass2(const FftPlan &p, const SpinArgs &s, hipStream_t st)
+{
+    IO_SpinP2<T> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1};
+    const int tc2 = FftCfg<T>::ELEMS / p.n1 / 2;
+    return launch_pass<T, -1>(p.n1, io, dim3(p.n2 / tc2, s.nslots),
+                              (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1], st);
+}
+
+template <typename T>
+hipError_t launch_fstat_blocks(const void *FT, void *mu, const FstatArgs &a, int nslots, hipStream_t st)
+{
+    constexpr int TRIG_TB = TrigCfg<T>::TB;
+    const int ntile = (a.nblk * a.nav + TRIG_TB - 1) / TRIG_TB;
+    if (ntile <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fstat_blocks<T>, dim3(ntile, nslots), dim3(256), 0, st, (const T *)FT, (T *)mu, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_triggers(const void *FT, const void *mu, const TrigArgs &a, int nslots, TrigEntry *out,
+                           int *counters, hipStream_t st)
+{
+    constexpr int TRIG_TB = TrigCfg<T>::TB;
+    const int ntile = (a.fr1 - a.fr0 + 1 + TRIG_TB - 1) / TRIG_TB;
+    if (ntile <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_triggers<T>, dim3(ntile, nslots), dim3(256), 0, st, (const T *)FT, (const T *)mu, a, out,
+                       counters);
+    return hipGetLastError();
+}
+
+#define TDF_INST(T)                                                                                        \
+    template hipError_t launch_sky_prep<T>(const SkyPrepArgs &, void *, int, int, hipStream_t);           \
+    template hipError_t launch_fft_natural_to_T<T>(const FftPlan &, const void *, void *, void *, int, int, \
+                                                   int, int, double, hipStream_t);                         \
+    template hipError_t launch_transpose<T>(const FftPlan &, const void *, void *, int, size_t, size_t, int,  \
+                                            int, int, hipStream_t);                                        \
+    template hipError_t launch_spline<T>(const void *, void *, int, size_t, size_t, int, const double *,     \
+                                         hipStream_t);                                                     \
+    template hipError_t launch_splint<T>(const void *, const void *, const double *, const double *, void *, \
+                                         int, int, int, size_t, size_t, int, hipStream_t);                 \
+    template hipError_t launch_spin_pass1<T>(const FftPlan &, const SpinArgs &, hipStream_t);              \
+    template hipError_t launch_spin_pass2<T>(const FftPlan &, const SpinArgs &, hipStream_t);              \
+    template hipError_t launch_fstat_blocks<T>(const void *, void *, const FstatArgs &, int, hipStream_t);  \
+    template hipError_t launch_triggers<T>(const void *, const void *, const TrigArgs &, int, TrigEntry *,  \
+                                           int *, hipStream_t);
+
+TDF_INST(float)
+TDF_INST(double)
+
+}  // namespace tdf

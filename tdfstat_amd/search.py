@@ -1,0 +1,201 @@
+"""Python host driver over the C ABI (mirror of job_core / search).
+
+``Context`` plays the role of plan_fftw + the global ifo[] of the reference
+(init.c:710-755, main.c:29): it owns the device copies of every detector's
+segment.  ``Context.job_core`` is job_core (jobcore.c:193-630) for one sky
+point; ``search`` is the sky loop of search() (jobcore.c:88-158) returning
+records instead of writing them to HDF5.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from . import settings as S
+
+
+@dataclass
+class JobOpts:
+    """The fields of Command_line_opts / Search_range that job_core reads."""
+    fr0: int
+    fr1: int
+    thr: float = 20.0
+    gtype_allsky: bool = False
+    sst: float = 0.0
+    spndr1: float = 0.0
+    sstep: float = 1.0
+    fstat_norm: bool = False
+    nav: int = S.NAVFSTAT
+    lines: tuple = ()
+    bufsize: int = 0
+
+    def to_c(self):
+        self._lines = np.ascontiguousarray(np.asarray(self.lines, dtype=np.float64).reshape(-1, 2)
+                                           if len(self.lines) else np.zeros((0, 2)))
+        o = L.TdfJobOpts()
+        o.gtype_allsky = int(self.gtype_allsky)
+        o.sst, o.spndr1, o.sstep = self.sst, self.spndr1, self.sstep
+        o.fr0, o.fr1, o.thr = self.fr0, self.fr1, self.thr
+        o.fstat_norm, o.nav = int(self.fstat_norm), self.nav
+        o.numlines = self._lines.shape[0]
+        o.lines = L.dptr(self._lines) if o.numlines else C.POINTER(C.c_double)()
+        o.bufsize = self.bufsize
+        return o
+
+
+def default_opts(sett, thr=20.0, narrowdown=None, overlap=0.03125, allsky=False, **kw):
+    """Options as main.c/init.c would set them: fr = [nmin, nmax]
+    (init.c:564-565), the two narrowdown edge lines (main.c:82) and
+    bufsize = nfftf/16 in allsky mode else nfftf (jobcore.c:59-65)."""
+    nd = S.default_narrowdown(overlap) if narrowdown is None else narrowdown
+    o = JobOpts(fr0=sett.nmin, fr1=sett.nmax, thr=thr, gtype_allsky=allsky,
+                lines=tuple(S.narrow_down_band(nd)),
+                bufsize=sett.nfftf // 16 if allsky else sett.nfftf)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def geom_from(sett):
+    g = L.TdfGeom()
+    g.N, g.nfft, g.Ninterp, g.nfftf = sett.N, sett.nfft, sett.Ninterp, sett.nfftf
+    g.interpftpad, g.nifo = sett.interpftpad, sett.nifo
+    g.oms, g.omr, g.dt = sett.oms, sett.omr, sett.dt
+    g.sepsm, g.cepsm = sett.sepsm, sett.cepsm
+    g.Smin, g.Smax = sett.Smin, sett.Smax
+    for k in range(16):
+        g.M[k] = float(sett.M[k])
+    return g
+
+
+@dataclass
+class Record:
+    m: float
+    n: float
+    s: float
+    ra: float
+    dec: float
+    fdot: float
+    ffstat: np.ndarray  # [ntrig, 2] (f, F) pairs
+
+
+class Context:
+    def __init__(self, sett, dets, prec=L.TDF_PREC_FP32, batch=0):
+        self.lib = L.load()
+        self.sett = sett
+        self.prec = prec
+        self._g = geom_from(sett)
+        h = C.c_void_p()
+        L.check(self.lib.tdf_ctx_create(C.byref(self._g), prec, batch, C.byref(h)), "tdf_ctx_create")
+        self.h = h
+        for k, d in enumerate(dets):
+            x = np.ascontiguousarray(d.xDat, dtype=np.float32)
+            ssb = np.ascontiguousarray(d.DetSSB, dtype=np.float64)
+            amod = np.ascontiguousarray(d.amod, dtype=np.float64)
+            L.check(self.lib.tdf_ctx_set_detector(h, k, L.fptr(x), L.dptr(ssb), d.sig2,
+                                                  math.cos(d.phir), math.sin(d.phir), L.dptr(amod)),
+                    "tdf_ctx_set_detector")
+
+    def close(self):
+        if self.h:
+            self.lib.tdf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- job_core pieces --
+    def sky_stage(self, pm, mm, nn, opts):
+        o = opts.to_c()
+        sky = L.TdfSky()
+        L.check(self.lib.tdf_sky_stage(self.h, pm, mm, nn, C.byref(o), C.byref(sky)), "tdf_sky_stage")
+        return sky
+
+    def spindown(self, iss_begin, iss_count, opts, pair_cap=None):
+        o = opts.to_c()
+        recs = (L.TdfRecord * max(1, iss_count))()
+        if pair_cap is None:
+            pair_cap = max(1024, iss_count * max(16, opts.bufsize // 2))
+        pairs = np.zeros(2 * pair_cap, dtype=np.float32)
+        npairs = C.c_longlong(0)
+        L.check(self.lib.tdf_spindown(self.h, iss_begin, iss_count, C.byref(o), recs, L.fptr(pairs),
+                                      pair_cap, C.byref(npairs)), "tdf_spindown")
+        return _records(recs, iss_count, pairs)
+
+    def job_core(self, pm, mm, nn, opts, max_spindowns=4096, pair_cap=None):
+        """jobcore.c:193-630 for one sky point -> (sky, [Record])."""
+        o = opts.to_c()
+        sky = L.TdfSky()
+        recs = (L.TdfRecord * max_spindowns)()
+        if pair_cap is None:
+            pair_cap = max(1024, max_spindowns * 64)
+        pairs = np.zeros(2 * pair_cap, dtype=np.float32)
+        nrec = C.c_int(0)
+        npairs = C.c_longlong(0)
+        L.check(self.lib.tdf_job_core(self.h, pm, mm, nn, C.byref(o), C.byref(sky), recs, max_spindowns,
+                                      L.fptr(pairs), pair_cap, C.byref(nrec), C.byref(npairs)),
+                "tdf_job_core")
+        return sky, _records(recs, nrec.value, pairs)
+
+    # -- inspection --
+    def sky_arrays(self, det):
+        N = self.sett.N
+        tmp1, shftf = np.zeros(N), np.zeros(N)
+        xa, xb = np.zeros(2 * N), np.zeros(2 * N)
+        L.check(self.lib.tdf_debug_sky_arrays(self.h, det, L.dptr(tmp1), L.dptr(shftf), L.dptr(xa),
+                                              L.dptr(xb)), "tdf_debug_sky_arrays")
+        return tmp1, shftf, xa.view(np.complex128), xb.view(np.complex128)
+
+    def fstat(self, iss):
+        F = np.zeros(self.sett.nfftf)
+        L.check(self.lib.tdf_debug_fstat(self.h, iss, L.dptr(F)), "tdf_debug_fstat")
+        return F
+
+    def profile(self, on=True):
+        L.check(self.lib.tdf_profile_enable(self.h, int(on)), "tdf_profile_enable")
+
+    def kernel_times(self):
+        out = {}
+        for slot in range(32):
+            ms = C.c_double(0)
+            cnt = C.c_longlong(0)
+            name = C.c_char_p()
+            if self.lib.tdf_kernel_times(self.h, slot, C.byref(ms), C.byref(cnt), C.byref(name)) != 0:
+                break
+            if cnt.value:
+                out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def synchronize(self):
+        L.check(self.lib.tdf_synchronize(self.h), "tdf_synchronize")
+
+
+def _records(recs, n, pairs):
+    out = []
+    for k in range(n):
+        r = recs[k]
+        ff = pairs[2 * r.offset: 2 * (r.offset + r.ntrig)].reshape(-1, 2).copy()
+        out.append(Record(r.m, r.n, r.s, r.ra, r.dec, r.fdot, ff))
+    return out
+
+
+def search(ctx, sky_points, opts):
+    """Sky loop of search() (jobcore.c:88-158): sky_points is an iterable of
+    (pm, mm, nn).  Returns (records, FNum)."""
+    records, fnum = [], 0
+    for pm, mm, nn in sky_points:
+        sky, recs = ctx.job_core(pm, mm, nn, opts)
+        records.extend(recs)
+        fnum += len(recs)
+    return records, fnum

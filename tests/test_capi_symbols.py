@@ -1,0 +1,33 @@
+"""The C-ABI library loads on a machine without a GPU and exports every
+function declared in include/tdfstat_hip.h (no compute calls here)."""
+import ctypes
+import os
+import re
+
+from tdfstat_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "tdfstat_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(tdf_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_api():
+    names = _declared()
+    assert "tdf_job_core" in names and "tdf_ctx_create" in names
+    assert set(names) == set(L.SIGNATURES), set(names) ^ set(L.SIGNATURES)
+
+
+def test_library_exports_every_symbol():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_strerror_without_gpu():
+    lib = L.load()
+    assert lib.tdf_strerror(L.TDF_OK) == b"ok"
+    assert b"Too many triggers" in lib.tdf_strerror(-4)
