@@ -1,0 +1,267 @@
+"""Throughput of the TDFstat coherent-search hot path on one MI355X.
+
+A step is one sky point of job_core: the per-sky stage plus its full
+spindown loop (S templates), on a seeded synthetic segment that is already
+resident in HBM.  value = templates/s (sky x spindown) of the FP32-bulk /
+FP64-phase path; the plain FP64 path, the FP32-vs-FP64 F-statistic error and
+a CPU baseline (the oracle, reference PREC=FLOAT semantics, OpenMP) are
+measured in the same run.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c5|c1]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Workloads of the reference configurations (realised odd x odd grids).
+CONFIGS = {
+    # 2 detectors, 2 days (N=86164), nfft=2^17, 31x33 sky x 255 spindowns
+    "c2": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(31, 33), sky_step=0.15, nspin=255,
+               injections=4, gaps=None, sigmas=None),
+    # 3 detectors, 6 days (N=258492), nfft=2^18, fftpad=2 -> nfftf=2^19,
+    # 99x101 sky x 1023 spindowns, 20% gaps
+    "c5": dict(dets=("H1", "L1", "V1"), nod=6, fftpad=2, sky=(99, 101), sky_step=0.045, nspin=1023,
+               injections=1, gaps=(0.2, 0.2, 0.2), sigmas=None),
+    # 2 detectors, 2 days, 3x5 sky x 33 spindowns, one injection
+    "c1": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(3, 5), sky_step=1.0, nspin=33,
+               injections=1, gaps=None, sigmas=None),
+}
+
+
+def sky_points(cfg, sett):
+    """Directed grid centred on (mm, nn) = (0, 0): odd x odd points with
+    fractional steps (init.c:649-682), visible points only, hemisphere 1."""
+    nm, nn_ = cfg["sky"]
+    st = cfg["sky_step"]
+    M = sett.M
+    pts = []
+    for a in range(nm):
+        for b in range(nn_):
+            mm = np.float32((a - nm // 2) * st)
+            nn = np.float32((b - nn_ // 2) * st)
+            al1 = float(nn) * M[10] + float(mm) * M[14]
+            al2 = float(nn) * M[11] + float(mm) * M[15]
+            if (al1 * al1 + al2 * al2) / sett.oms ** 2 <= 1.0:
+                pts.append((1, float(mm), float(nn)))
+    return pts
+
+
+def make_workload(name, seed=20261203):
+    from tdfstat_amd import synth
+    cfg = CONFIGS[name]
+    rng = np.random.default_rng(seed)
+    injs = [synth.random_injection(rng, float(rng.uniform(8, 15))) for _ in range(cfg["injections"])]
+    seg = synth.make_segment(nod=cfg["nod"], det_names=cfg["dets"], fftpad=cfg["fftpad"], seed=seed,
+                             gap_fracs=cfg["gaps"], sigmas=cfg["sigmas"], injections=injs)
+    return cfg, seg
+
+
+def algorithmic_bytes(sett, S, prec64):
+    """Bytes per template (B_spin + B_sky/S) of the algorithmic model:
+    B_spin = 32*nfftf + nifo*N*24/S (FP32 bulk) or 64*nfftf + nifo*N*40/S,
+    B_sky  = nifo*(52*N + 128*nfft) or nifo*(68*N + 256*nfft)."""
+    nifo, N, nfft, nfftf = sett.nifo, sett.N, sett.nfft, sett.nfftf
+    if prec64:
+        return 64 * nfftf + nifo * N * 40 / S + nifo * (68 * N + 256 * nfft) / S
+    return 32 * nfftf + nifo * N * 24 / S + nifo * (52 * N + 128 * nfft) / S
+
+
+def kernel_bytes(sett, S, nslots, kname, prec64):
+    """Algorithmic bytes of one launch of a spindown kernel over nslots
+    templates of one sky point."""
+    c = 16 if prec64 else 8
+    r = 8 if prec64 else 4
+    nF = sett.nmax - sett.nmin + 1
+    nifo, N, nfftf = sett.nifo, sett.N, sett.nfftf
+    if kname == "spin_pass1_demod_fft":
+        # write Fa, Fb (2 x nfftf complex) per template; per-sky inputs
+        # (xDatma, xDatmb complex + FP64 phase base per sample) once per launch
+        return nslots * 2 * nfftf * c + nifo * N * (2 * c + 8)
+    if kname == "spin_pass2_fft_fstat":
+        return nslots * (2 * nfftf * c + nF * r)
+    if kname == "triggers":
+        return nslots * nF * r
+    return None
+
+
+def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0):
+    from tdfstat_amd import search as TS
+    sett = seg.sett
+    S = cfg["nspin"]
+    opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
+    ctx = TS.Context(sett, seg.dets, prec=prec, batch=batch)
+    oc = opts.to_c()
+    cap = (S + 1) * 64
+    for k in range(warmup):
+        ctx.job_core_raw(*pts[k % len(pts)], oc, S + 1, cap)
+    ctx.synchronize()
+    if profile:
+        ctx.profile(True)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nt = 0
+    ntrig = 0
+    for k in range(steps):
+        sky, nrec, npairs = ctx.job_core_raw(*pts[(warmup + k) % len(pts)], oc, S + 1, cap)
+        nt += nrec
+        ntrig += npairs
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kt = ctx.kernel_times() if profile else {}
+    return ctx, opts, nt, el, kt, ntrig
+
+
+def fp32_vs_fp64_error(cfg, seg, pt, opts, ctx32, ctx64, nss=8):
+    """max / mean relative F error of the FP32 path vs the FP64 path over
+    bins with F64 >= 2, and trigger-set equality, on nss spindowns."""
+    S = cfg["nspin"]
+    sky32, r32 = ctx32.job_core(*pt, opts, max_spindowns=S + 1)
+    sky64, r64 = ctx64.job_core(*pt, opts, max_spindowns=S + 1)
+    errs = []
+    last0 = (S - 1) - ((S - 1) % ctx32.batch)
+    for iss in range(last0, min(S, last0 + nss)):
+        a, b = ctx32.fstat(iss), ctx64.fstat(iss)
+        sel = b >= 2.0
+        errs.append(np.abs(a[sel] - b[sel]) / b[sel])
+    e = np.concatenate(errs) if errs else np.zeros(1)
+    thr = opts.thr
+    same = True
+    for x, y in zip(r32, r64):
+        fx = {round(float(f), 6) for f, v in x.ffstat if abs(v - thr) > 1e-3 * thr}
+        fy = {round(float(f), 6) for f, v in y.ffstat if abs(v - thr) > 1e-3 * thr}
+        same &= fx == fy
+    return float(e.max()), float(e.mean()), bool(same)
+
+
+def cpu_baseline(cfg, seg, pts, budget_s=12.0):
+    """Oracle job_core, reference PREC=FLOAT semantics, OpenMP over
+    spindowns on this host's cores; bounded sample of sky points."""
+    from oracle import oracle as O
+    from tdfstat_amd import search as TS
+    sett = seg.sett
+    S = cfg["nspin"]
+    opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(ncpu, 16))
+    orc = O.Oracle(sett, seg.dets)
+    orc.job_core(*pts[0], opts, prec=O.PREC_FLOAT, iss_count=min(S, 2 * threads), nthreads=threads)  # warm
+    t0 = time.perf_counter()
+    nt, nsky = 0, 0
+    while True:
+        sky, n, _ = orc.job_core(*pts[nsky % len(pts)], opts, prec=O.PREC_FLOAT, nthreads=threads)
+        nt += sky.iss_size
+        nsky += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or nsky >= 64:
+            break
+    return dict(value=nt / el, unit="templates/s", cores=threads, kind="port",
+                sample=f"{nsky} sky point(s) x {S} spindowns ({nt} templates) of the same workload, "
+                       f"oracle job_core with PREC=FLOAT semantics, OpenMP over spindowns, "
+                       f"{threads} threads, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--batch", type=int, default=0, help="spindowns per launch (0 = library default)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a plain 256 MiB device copy first (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the search shards by sky point across GPUs, see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "no GPU"
+    from tdfstat_amd import _lib as L
+    if args.calib_copy:
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)      # 256 MiB read + 256 MiB write (blit)
+        s_ = src.sum()          # 256 MiB read (reduction kernel)
+        torch.cuda.synchronize()
+        del src, dst
+
+    cfg, seg = make_workload(args.config)
+    sett = seg.sett
+    pts = sky_points(cfg, sett)
+    S = cfg["nspin"]
+    print(f"[bench] config={args.config} N={sett.N} nfft={sett.nfft} nfftf={sett.nfftf} nifo={sett.nifo} "
+          f"sky={len(pts)} S={S}", file=sys.stderr, flush=True)
+
+    ctx32, opts, nt, el, kt, ntrig = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
+                                             args.batch)
+    value = nt / el
+    print(f"[bench] fp32: {nt} templates in {el:.3f} s -> {value:.0f} templates/s", file=sys.stderr, flush=True)
+    k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 4)
+    ctx64, _, nt64, el64, kt64, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch)
+    value64 = nt64 / el64
+    print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
+          flush=True)
+    emax, emean, same = fp32_vs_fp64_error(cfg, seg, pts[0], opts, ctx32, ctx64)
+
+    # roofline of the dominant spindown kernel (HIP events on the library stream)
+    spin = {k: v for k, v in kt.items() if k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat", "triggers")}
+    dom = max(spin, key=lambda k: spin[k][0])
+    ms_tot, launches = spin[dom]
+    nslots_avg = nt / max(1, kt["spin_pass1_demod_fft"][1])
+    kb = kernel_bytes(sett, S, nslots_avg, dom, False)
+    ach = kb / (ms_tot / launches * 1e-3) / 1e9
+    bt = algorithmic_bytes(sett, S, False)
+    pipe = value * bt / 1e9
+    line = {
+        "metric": "templates/s (sky x spindown), FP32 bulk + FP64 phase",
+        "value": round(value, 1),
+        "unit": "templates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: nifo={sett.nifo} N={sett.N} nfft={sett.nfft} "
+                               f"nfftf={sett.nfftf} {len(pts)} sky x {S} spindowns, seeded Gaussian "
+                               f"noise + injections, FP64 phase",
+                   "templates_timed": nt, "triggers": ntrig},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "avg_launch_ms": round(ms_tot / launches, 5),
+                     "bytes_per_launch": int(kb)},
+        "pipeline_roofline": {"bytes_per_template": int(bt), "achieved": round(pipe, 1),
+                              "frac": round(pipe / HBM_PEAK_GBS, 4)},
+        "fp64_path": {"value": round(value64, 1), "speedup_fp32_over_fp64": round(value / value64, 3)},
+        "fp32_vs_fp64": {"max_rel_err_F": emax, "mean_rel_err_F": emean, "trigger_sets_equal": same},
+        "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(cfg, seg, pts)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

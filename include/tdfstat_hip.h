@@ -143,6 +143,10 @@ int tdf_job_core(tdf_ctx *ctx, int pm, float mm, float nn,
  * Any pointer may be NULL. */
 int tdf_debug_sky_arrays(tdf_ctx *ctx, int det, double *tmp1, double *shftf,
                          double *xDatma, double *xDatmb);
+/* interpolated series y (after the inverse FFT, natural order) and its
+ * spline second derivatives y2 for item = 2*det + (0: a, 1: b), first
+ * interpftpad*N nodes, as interleaved complex doubles. */
+int tdf_debug_sky_interp(tdf_ctx *ctx, int item, double *y, double *y2);
 /* F-statistic (after optional FStat normalisation) of spindown
  * iss of the last tdf_spindown call, natural bin order, length nfftf;
  * bins outside [fr0, fr1] are 0. */

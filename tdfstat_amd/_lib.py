@@ -73,6 +73,7 @@ SIGNATURES = {
                                C.POINTER(C.c_longlong)]),
     "tdf_debug_sky_arrays": (C.c_int, [_P, C.c_int, _DP, _DP, _DP, _DP]),
     "tdf_debug_fstat": (C.c_int, [_P, C.c_int, _DP]),
+    "tdf_debug_sky_interp": (C.c_int, [_P, C.c_int, _DP, _DP]),
     "tdf_fft_c2c": (C.c_int, [_DP, _DP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "tdf_profile_enable": (C.c_int, [_P, C.c_int]),
     "tdf_kernel_times": (C.c_int, [_P, C.c_int, _DP, C.POINTER(C.c_longlong),

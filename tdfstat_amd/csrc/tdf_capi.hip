@@ -85,14 +85,19 @@ struct tdf_ctx {
     // spindown batch buffers
     void *Z = nullptr, *FT = nullptr, *mu = nullptr;
     int nblk_cap = 0;
-    TmplDesc *desc_d = nullptr, *desc_h = nullptr;
     TrigEntry *trig_d = nullptr, *trig_h = nullptr;
+    // counters: [0] triggers, [1..S] per template, then one candidate count
+    // per batch; grown on demand
     int *cnt_d = nullptr, *cnt_h = nullptr;
+    int cnt_cap = 0;
+    TrigEntry *cand_d = nullptr;
+    long long cand_cap = 0;
     long long trig_cap = 0;
     double *lines_d = nullptr;
-    int lines_cap = 0;
-    int last_batch_n = 0, last_batch_iss0 = 0;
-    int last_fr0 = 0, last_fr1 = -1;
+    int lines_cap = 0, lines_n = -1;
+    std::vector<double> lines_h;
+    int last_batch_n = 0, last_batch_iss0 = 0, call_iss0 = 0;
+    int last_fr0 = 0, last_fr1 = -1, last_nblk = 0, last_nav = 1;
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev;
@@ -278,58 +283,98 @@ int check_opts(tdf_ctx *c, const tdf_job_opts *o)
     return TDF_OK;
 }
 
+// Launches the spindown kernels for templates [t0, t0+count) of the current
+// call (descriptors already on the device): pass 1, pass 2, FStat blocks and
+// triggers for each batch of up to c->batch templates.  No host sync.
 template <typename T>
-int run_spindown_batch(tdf_ctx *c, int nslots, const tdf_job_opts *o)
+int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, const SkyScal &sk,
+                          bool use_cand, int *cnt_dev, int ncnt_tmpl)
 {
     hipStream_t st = c->stream;
     const FftPlan &p = c->p_spin;
     const tdf_geom &g = c->g;
-    HIPCHK(hipMemcpyAsync(c->desc_d, c->desc_h, sizeof(TmplDesc) * nslots, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(c->cnt_d, 0, sizeof(int) * (1 + c->batch), st));
-    SpinArgs sa{c->xm, c->tmp1, c->desc_d, c->Z, c->FT, c->sky_scal, g.N, g.nifo, 0, nslots, o->fr0, o->fr1};
-    {
-        Prof pr(c, KS_SPIN1);
-        HIPCHK(launch_spin_pass1<T>(p, sa, st));
-    }
-    {
-        Prof pr(c, KS_SPIN2);
-        HIPCHK(launch_spin_pass2<T>(p, sa, st));
-    }
     const int nblk = o->fstat_norm ? (o->fr1 - o->fr0) / o->nav : 0;
-    if (o->fstat_norm && nblk > 0) {
-        FstatArgs fa{p.n, p.n1, p.n2, o->fr0, o->fr1, nblk, o->nav, c->nblk_cap};
-        Prof pr(c, KS_FSTAT);
-        HIPCHK(launch_fstat_blocks<T>(c->FT, c->mu, fa, nslots, st));
+    int batch_idx = 0;
+    for (int b0 = t0; b0 < t0 + count; b0 += c->batch, batch_idx++) {
+        const int nslots = std::min(c->batch, t0 + count - b0);
+        int *cand_cnt = cnt_dev + 1 + ncnt_tmpl + batch_idx;
+        SpinArgs sa{};
+        sa.xm = c->xm;
+        sa.tmp1 = c->tmp1;
+        sa.sk = sk;
+        sa.Z = c->Z;
+        sa.FT = c->FT;
+        sa.sky = c->sky_scal;
+        sa.N = g.N;
+        sa.nifo = g.nifo;
+        sa.slot0 = b0;
+        sa.nslots = nslots;
+        sa.fr0 = o->fr0;
+        sa.fr1 = o->fr1;
+        sa.emit_cand = use_cand ? 1 : 0;
+        sa.thr = o->thr;
+        sa.cand = c->cand_d;
+        sa.cand_cnt = cand_cnt;
+        sa.cand_cap = (int)c->cand_cap;
+        {
+            Prof pr(c, KS_SPIN1);
+            HIPCHK(launch_spin_pass1<T>(p, sa, st));
+        }
+        {
+            Prof pr(c, KS_SPIN2);
+            HIPCHK(launch_spin_pass2<T>(p, sa, st));
+        }
+        if (!use_cand && o->fstat_norm && nblk > 0) {
+            FstatArgs fa{p.n, p.n1, p.n2, p.l2, o->fr0, o->fr1, nblk, o->nav, c->nblk_cap};
+            Prof pr(c, KS_FSTAT);
+            HIPCHK(launch_fstat_blocks<T>(c->FT, c->mu, fa, nslots, st));
+        }
+        TrigArgs ta{};
+        ta.n = p.n;
+        ta.n1 = p.n1;
+        ta.n2 = p.n2;
+        ta.l2 = p.l2;
+        ta.slot0 = b0;
+        ta.nfftf = g.nfftf;
+        ta.fr0 = o->fr0;
+        ta.fr1 = o->fr1;
+        ta.thr = o->thr;
+        ta.fstat_norm = (o->fstat_norm && nblk > 0) ? 1 : 0;
+        ta.nblk = nblk;
+        ta.nav = o->nav > 0 ? o->nav : 1;
+        ta.nblk_cap = c->nblk_cap;
+        ta.numlines = o->numlines;
+        ta.cap = (int)std::min<long long>(c->trig_cap, 0x7fffffff);
+        ta.lines = c->lines_d;
+        ta.sk = sk;
+        if (use_cand) {
+            Prof pr(c, KS_TRIG);
+            HIPCHK(launch_candidates<T>(c->FT, ta, c->cand_d, cand_cnt, (int)c->cand_cap, c->trig_d, cnt_dev, st));
+        } else {
+            Prof pr(c, KS_TRIG);
+            HIPCHK(launch_triggers<T>(c->FT, c->mu, ta, nslots, c->trig_d, cnt_dev, st));
+        }
+        c->last_batch_n = nslots;
+        c->last_batch_iss0 = c->call_iss0 + b0;
     }
-    TrigArgs ta{};
-    ta.n = p.n;
-    ta.n1 = p.n1;
-    ta.n2 = p.n2;
-    ta.nfftf = g.nfftf;
-    ta.fr0 = o->fr0;
-    ta.fr1 = o->fr1;
-    ta.thr = o->thr;
-    ta.fstat_norm = (o->fstat_norm && nblk > 0) ? 1 : 0;
-    ta.nblk = nblk;
-    ta.nav = o->nav > 0 ? o->nav : 1;
-    ta.nblk_cap = c->nblk_cap;
-    ta.numlines = o->numlines;
-    ta.cap = (int)std::min<long long>(c->trig_cap, 0x7fffffff);
-    ta.lines = c->lines_d;
-    ta.desc = c->desc_d;
-    {
-        Prof pr(c, KS_TRIG);
-        HIPCHK(launch_triggers<T>(c->FT, c->mu, ta, nslots, c->trig_d, c->cnt_d, st));
-    }
-    HIPCHK(hipMemcpyAsync(c->cnt_h, c->cnt_d, sizeof(int) * (1 + nslots), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    const long long ntot = c->cnt_h[0];
-    if (ntot > 0) {
-        const long long ncp = std::min(ntot, c->trig_cap);
-        HIPCHK(hipMemcpyAsync(c->trig_h, c->trig_d, sizeof(TrigEntry) * ncp, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-    }
-    prof_collect(c);
+    return TDF_OK;
+}
+
+// Grows the per-call counter buffers to hold n templates (+ batch counters).
+int ensure_counters(tdf_ctx *c, int n)
+{
+    const int need = 2 + n + (n + c->batch - 1) / c->batch;
+    if (need <= c->cnt_cap) return TDF_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int cap = std::max(need, 2 * c->cnt_cap);
+    if (c->cnt_d) (void)hipFree(c->cnt_d);
+    if (c->cnt_h) (void)hipHostFree(c->cnt_h);
+    c->cnt_d = nullptr;
+    c->cnt_h = nullptr;
+    c->cnt_cap = 0;
+    if (hipMalloc((void **)&c->cnt_d, sizeof(int) * cap) != hipSuccess) return TDF_ERR_NOMEM;
+    if (hipHostMalloc((void **)&c->cnt_h, sizeof(int) * cap, 0) != hipSuccess) return TDF_ERR_NOMEM;
+    c->cnt_cap = cap;
     return TDF_OK;
 }
 
@@ -364,7 +409,7 @@ int tdf_ctx_destroy(tdf_ctx *c)
     if (!c) return TDF_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : c->allocs) (void)hipFree(p);
-    if (c->desc_h) (void)hipHostFree(c->desc_h);
+    if (c->cnt_d) (void)hipFree(c->cnt_d);
     if (c->trig_h) (void)hipHostFree(c->trig_h);
     if (c->cnt_h) (void)hipHostFree(c->cnt_h);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -449,13 +494,12 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
     TRY(dmalloc(c, &c->FT, c->rsz * B * (size_t)g->nfftf));
     c->nblk_cap = g->nfftf / 16 + 1;  // nav >= 16 in practice; checked per call
     TRY(dmalloc(c, &c->mu, c->rsz * B * (size_t)c->nblk_cap));
-    TRY(dmalloc(c, (void **)&c->desc_d, sizeof(TmplDesc) * B));
-    if (hipHostMalloc((void **)&c->desc_h, sizeof(TmplDesc) * B, 0) != hipSuccess) TRY(TDF_ERR_NOMEM);
-    TRY(dmalloc(c, (void **)&c->cnt_d, sizeof(int) * (1 + B)));
-    if (hipHostMalloc((void **)&c->cnt_h, sizeof(int) * (1 + B), 0) != hipSuccess) TRY(TDF_ERR_NOMEM);
+    TRY(ensure_counters(c, std::max(4096, (int)B)));
     // every record may hold up to nfftf/2 pairs (bufsize <= nfftf)
     c->trig_cap = std::min<long long>((long long)B * (g->nfftf / 2 + 1), 1LL << 24);
     TRY(dmalloc(c, (void **)&c->trig_d, sizeof(TrigEntry) * c->trig_cap));
+    c->cand_cap = std::min<long long>((long long)B * 4096, 1LL << 22);
+    TRY(dmalloc(c, (void **)&c->cand_d, sizeof(TrigEntry) * c->cand_cap));
     if (hipHostMalloc((void **)&c->trig_h, sizeof(TrigEntry) * c->trig_cap, 0) != hipSuccess) TRY(TDF_ERR_NOMEM);
     c->lines_cap = 512;
     TRY(dmalloc(c, (void **)&c->lines_d, sizeof(double) * 2 * c->lines_cap));
@@ -536,7 +580,6 @@ int tdf_sky_stage(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o,
     a.cosdel = cosdel;
     for (int d = 0; d < g.nifo; d++) {
         const DetParams &dp = c->detp_h[d];
-        a.shft1[d] = a.ns0 * c->ssb0_h[3 * d] + a.ns1 * c->ssb0_h[3 * d + 1] + a.ns2 * c->ssb0_h[3 * d + 2];
         a.cosalfr[d] = cosal * dp.cphir + sinal * dp.sphir;
         a.sinalfr[d] = sinal * dp.cphir - cosal * dp.sphir;
     }
@@ -581,63 +624,104 @@ int tdf_spindown(tdf_ctx *c, int iss_begin, int iss_count, const tdf_job_opts *o
     if (!c->sky_valid || !c->sky.visible) return TDF_ERR_STATE;
     if (o->numlines > c->lines_cap) return TDF_ERR_UNSUPPORTED;
     if (o->fstat_norm && (o->fr1 - o->fr0) / o->nav > c->nblk_cap) return TDF_ERR_UNSUPPORTED;
+    if ((rc = ensure_counters(c, std::max(iss_count, 1)))) return rc;
     const tdf_geom &g = c->g;
     const double *M = g.M;
-    const float mm = c->mm, nn = c->nn;
-    if (o->numlines > 0)
-        HIPCHK(hipMemcpyAsync(c->lines_d, o->lines, sizeof(double) * 2 * o->numlines, hipMemcpyHostToDevice,
-                              c->stream));
-    long long np = 0;
+    SkyScal sk{c->sky.smin, o->sstep, c->mm, c->nn, c->sky.het0, M[0], M[4], M[5], M[9], M[13], iss_begin};
+    if (o->numlines > 0 && (o->numlines != c->lines_n ||
+                            memcmp(c->lines_h.data(), o->lines, sizeof(double) * 2 * o->numlines) != 0)) {
+        c->lines_h.assign(o->lines, o->lines + 2 * o->numlines);
+        c->lines_n = o->numlines;
+        HIPCHK(hipMemcpyAsync(c->lines_d, c->lines_h.data(), sizeof(double) * 2 * o->numlines,
+                              hipMemcpyHostToDevice, c->stream));
+    }
+    // records (jobcore.c:437-456); the same scalars are derived on the device
+    for (int k = 0; k < iss_count; k++) {
+        const TmplDesc td = tmpl_scalars(sk, iss_begin + k);
+        const float ss = sk.smin + (float)(iss_begin + k) * sk.sstep;
+        tdf_record &r = recs[k];
+        r.m = c->mm;
+        r.n = c->nn;
+        r.s = ss;
+        r.ra = (float)c->sky.ra;
+        r.dec = (float)c->sky.dec;
+        r.fdot = (float)(td.spnd / (M_PI * g.dt * g.dt));
+        r.ntrig = 0;
+        r.offset = 0;
+    }
+    c->last_fr0 = o->fr0;
+    c->last_fr1 = o->fr1;
+    c->last_nblk = o->fstat_norm ? (o->fr1 - o->fr0) / o->nav : 0;
+    c->last_nav = o->nav > 0 ? o->nav : 1;
+    if (npairs) *npairs = 0;
+    if (iss_count == 0) return TDF_OK;
     const int maxtrig = o->bufsize / 2;
-    std::vector<std::vector<std::pair<int, std::pair<float, float>>>> per;
-    for (int b0 = 0; b0 < iss_count; b0 += c->batch) {
-        const int nslots = std::min(c->batch, iss_count - b0);
-        for (int k = 0; k < nslots; k++) {
-            const int iss = iss_begin + b0 + k;
-            // jobcore.c:430-468 (DOUBLE build arithmetic)
-            const float ss = c->sky.smin + iss * o->sstep;
-            const double spnd = ss * M[5] + nn * M[9] + mm * M[13];
-            double het1 = fmod(ss * M[4], M[0]);
-            if (het1 < 0) het1 += M[0];
-            c->desc_h[k] = TmplDesc{het1, spnd, c->sky.het0 + het1};
-            tdf_record &r = recs[b0 + k];
-            r.m = mm;
-            r.n = nn;
-            r.s = ss;
-            r.ra = (float)c->sky.ra;
-            r.dec = (float)c->sky.dec;
-            r.fdot = (float)(spnd / (M_PI * g.dt * g.dt));
-            r.ntrig = 0;
-            r.offset = 0;
-        }
-        rc = c->prec == TDF_PREC_FP64 ? run_spindown_batch<double>(c, nslots, o)
-                                      : run_spindown_batch<float>(c, nslots, o);
-        if (rc) return rc;
-        c->last_batch_n = nslots;
-        c->last_batch_iss0 = iss_begin + b0;
-        c->last_fr0 = o->fr0;
-        c->last_fr1 = o->fr1;
-        for (int k = 0; k < nslots; k++)
-            if (c->cnt_h[1 + k] > maxtrig) return TDF_ERR_TRIGGER_OVERFLOW;
-        const long long ntot = c->cnt_h[0];
-        if (ntot > c->trig_cap) return TDF_ERR_TRIGGER_OVERFLOW;
-        per.assign(nslots, {});
-        for (long long t = 0; t < ntot; t++) {
-            const TrigEntry &e = c->trig_h[t];
-            per[e.slot].push_back({e.ic, {e.f, e.F}});
-        }
-        for (int k = 0; k < nslots; k++) {
-            auto &v = per[k];
-            std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
-            tdf_record &r = recs[b0 + k];
-            r.ntrig = (int)v.size();
-            r.offset = np;
-            if (np + (long long)v.size() > pair_cap) return TDF_ERR_CAPACITY;
-            for (auto &e : v) {
-                pairs[2 * np] = e.second.first;
-                pairs[2 * np + 1] = e.second.second;
-                np++;
+    c->call_iss0 = iss_begin;
+    const int nbatch = (iss_count + c->batch - 1) / c->batch;
+    const int ncnt = 1 + iss_count + nbatch;
+    long long np = 0;
+    std::vector<std::vector<std::pair<int, std::pair<float, float>>>> per(iss_count);
+    // pass-2 candidates when F is not block-normalised; one readback per call;
+    // on overflow of a shared list, redo with the full trigger scan / per batch
+    bool use_cand = !o->fstat_norm || (o->fr1 - o->fr0) / std::max(o->nav, 1) == 0;
+    int chunk = iss_count;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        bool redo = false;
+        for (auto &v : per) v.clear();
+        for (int t0 = 0; t0 < iss_count && !redo; t0 += chunk) {
+            const int cnt = std::min(chunk, iss_count - t0);
+            HIPCHK(hipMemsetAsync(c->cnt_d, 0, sizeof(int) * ncnt, c->stream));
+            rc = c->prec == TDF_PREC_FP64
+                     ? launch_spindown_range<double>(c, t0, cnt, o, sk, use_cand, c->cnt_d, iss_count)
+                     : launch_spindown_range<float>(c, t0, cnt, o, sk, use_cand, c->cnt_d, iss_count);
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(c->cnt_h, c->cnt_d, sizeof(int) * ncnt, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int k = t0; k < t0 + cnt; k++)
+                if (c->cnt_h[1 + k] > maxtrig) {
+                    prof_collect(c);
+                    return TDF_ERR_TRIGGER_OVERFLOW;
+                }
+            if (use_cand)
+                for (int b = 0; b < (cnt + c->batch - 1) / c->batch; b++)
+                    if (c->cnt_h[1 + iss_count + b] > c->cand_cap) {
+                        use_cand = false;  // too many candidates: full scan
+                        redo = true;
+                    }
+            const long long ntot = c->cnt_h[0];
+            if (!redo && ntot > c->trig_cap) {
+                if (chunk == c->batch) {
+                    prof_collect(c);
+                    return TDF_ERR_CAPACITY;
+                }
+                chunk = c->batch;
+                redo = true;
             }
+            if (redo) break;
+            if (ntot > 0) {
+                HIPCHK(hipMemcpyAsync(c->trig_h, c->trig_d, sizeof(TrigEntry) * ntot, hipMemcpyDeviceToHost,
+                                      c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+            }
+            for (long long t = 0; t < ntot; t++) {
+                const TrigEntry &e = c->trig_h[t];
+                per[e.slot].push_back({e.ic, {e.f, e.F}});
+            }
+        }
+        prof_collect(c);
+        if (!redo) break;
+    }
+    for (int k = 0; k < iss_count; k++) {
+        auto &v = per[k];
+        std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        tdf_record &r = recs[k];
+        r.ntrig = (int)v.size();
+        r.offset = np;
+        if (np + (long long)v.size() > pair_cap) return TDF_ERR_CAPACITY;
+        for (auto &e : v) {
+            pairs[2 * np] = e.second.first;
+            pairs[2 * np + 1] = e.second.second;
+            np++;
         }
     }
     if (npairs) *npairs = np;
@@ -683,6 +767,27 @@ int tdf_debug_sky_arrays(tdf_ctx *c, int det, double *tmp1, double *shftf, doubl
     return TDF_OK;
 }
 
+int tdf_debug_sky_interp(tdf_ctx *c, int item, double *y, double *y2)
+{
+    if (!c || item < 0 || item >= 2 * c->g.nifo) return TDF_ERR_ARG;
+    if (!c->sky_valid) return TDF_ERR_STATE;
+    const size_t n = (size_t)c->g.interpftpad * c->g.N;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int w = 0; w < 2; w++) {
+        double *dst = w == 0 ? y : y2;
+        if (!dst) continue;
+        const char *src = (const char *)(w == 0 ? c->Ymv : c->Zsky) + item * (size_t)c->g.Ninterp * c->csz;
+        if (c->prec == TDF_PREC_FP64) {
+            HIPCHK(hipMemcpy(dst, src, 16 * n, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> h(2 * n);
+            HIPCHK(hipMemcpy(h.data(), src, 8 * n, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < 2 * n; i++) dst[i] = h[i];
+        }
+    }
+    return TDF_OK;
+}
+
 int tdf_debug_fstat(tdf_ctx *c, int iss, double *F)
 {
     if (!c || !F) return TDF_ERR_ARG;
@@ -703,6 +808,24 @@ int tdf_debug_fstat(tdf_ctx *c, int iss, double *F)
     // FT[k2*n1 + k1] = F[k2 + n2*k1]
     for (size_t k = 0; k < n; k++) F[k] = 0.;
     for (int k = c->last_fr0; k <= c->last_fr1; k++) F[k] = ft[(size_t)(k % p.n2) * p.n1 + k / p.n2];
+    if (c->last_nblk > 0) {
+        // FStat block means of this slot (auxi.c:336-363 applied as F / mu)
+        std::vector<double> mu(c->last_nblk);
+        const char *msrc = (const char *)c->mu + slot * (size_t)c->nblk_cap * c->rsz;
+        if (c->prec == TDF_PREC_FP64) {
+            HIPCHK(hipMemcpy(mu.data(), msrc, 8 * mu.size(), hipMemcpyDeviceToHost));
+            for (int b = 0; b < c->last_nblk; b++)
+                for (int k = 0; k < c->last_nav; k++) F[c->last_fr0 + b * c->last_nav + k] /= mu[b];
+        } else {
+            std::vector<float> h(mu.size());
+            HIPCHK(hipMemcpy(h.data(), msrc, 4 * h.size(), hipMemcpyDeviceToHost));
+            for (int b = 0; b < c->last_nblk; b++)
+                for (int k = 0; k < c->last_nav; k++) {
+                    float v = (float)F[c->last_fr0 + b * c->last_nav + k] / h[b];
+                    F[c->last_fr0 + b * c->last_nav + k] = v;
+                }
+        }
+    }
     return TDF_OK;
 }
 

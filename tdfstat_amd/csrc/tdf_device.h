@@ -26,6 +26,20 @@ __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b)
 }
 template <typename T>
 __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+// Streaming (non-temporal) access for data written once and read once by
+// the next kernel, so that it does not evict reused inputs from L2.
+template <typename T>
+__device__ __forceinline__ void st_stream(cx<T> *p, cx<T> v)
+{
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> ld_stream(const cx<T> *p)
+{
+    return cx<T>{__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y)};
+}
 template <typename T>
 __device__ __forceinline__ T cnorm(cx<T> a) { return a.x * a.x + a.y * a.y; }
 
@@ -149,12 +163,13 @@ __device__ __forceinline__ void dftR(cx<T> *v)
 }
 
 // ---------------------------------------------------------------------
-// LDS tile of NCOL columns, each L complex values; column stride LP with one
-// pad slot per 16 elements to break the power-of-two strides of the
-// Stockham writes.
+// LDS tile of NCOL columns, each L complex values.  Inside a column one pad
+// slot follows every 16 elements (breaks the power-of-two strides of the
+// Stockham writes); the column stride LP is odd (in complex elements) so that
+// 16 lanes touching 16 consecutive columns hit 32 distinct banks.
 template <int L>
 struct ldsgeom {
-    static constexpr int LP = L + L / 16;
+    static constexpr int LP = L + L / 16 + 1;
     __device__ static __forceinline__ int pad(int p) { return p + (p >> 4); }
 };
 

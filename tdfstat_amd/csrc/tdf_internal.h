@@ -57,7 +57,7 @@ struct SkyPrepArgs {
     double omr, oms, het0;
     double ns0, ns1, ns2;
     double c2d, c2sd, sindel, cosdel;
-    double shft1[MAXDET], cosalfr[MAXDET], sinalfr[MAXDET];
+    double cosalfr[MAXDET], sinalfr[MAXDET];
 };
 
 // Four-step factorisation n = n1 * n2: pass 1 = n1 columns of length n2,
@@ -68,30 +68,60 @@ struct FftPlan {
     const void *tw_lo[2], *tw_hi[2];
 };
 
+// Scalars of the current sky point from which every template's
+// (het1, spnd, sgnl0) is derived on the device (jobcore.c:430-468).
+struct SkyScal {
+    float smin, sstep, mm, nn;
+    double het0, M0, M4, M5, M9, M13;
+    int iss0;  // template index of slot 0 of the call
+};
+
 struct TmplDesc {
     double het1, spnd, sgnl0;
 };
 
+// jobcore.c:430,439,465-468 with the DOUBLE build's conversions and separate
+// roundings: ss = smin + iss*sstep (float), spnd = ss*M5 + nn*M9 + mm*M13,
+// het1 = fmod(ss*M4, M0) (+M0 if negative), sgnl0 = het0 + het1.
+__host__ __device__ inline TmplDesc tmpl_scalars(const SkyScal &k, int iss)
+{
+#pragma clang fp contract(off)
+    const float ss = k.smin + (float)iss * k.sstep;
+    const double spnd = (double)ss * k.M5 + (double)k.nn * k.M9 + (double)k.mm * k.M13;
+    double het1 = fmod((double)ss * k.M4, k.M0);
+    if (het1 < 0) het1 += k.M0;
+    return TmplDesc{het1, spnd, k.het0 + het1};
+}
+
+struct TrigEntry;
+
 struct SpinArgs {
     const void *xm;       // [nifo][2][N] complex T
     const double *tmp1;   // [nifo][N]
-    const TmplDesc *desc; // [nslots]
+    SkyScal sk;
     void *Z;              // [nslots][2][n]
     void *FT;             // [nslots][n]
     const double *sky;    // aa, bb
     int N, nifo, slot0, nslots, fr0, fr1;
+    // candidate emission from pass 2 (no FStat normalisation): bins
+    // fr0+1..fr1-2 with F >= thr go to cand[] via counters[0]
+    int emit_cand;
+    double thr;
+    TrigEntry *cand;
+    int *cand_cnt;
+    int cand_cap;
 };
 
 struct FstatArgs {
-    int n, n1, n2, fr0, fr1, nblk, nav, nblk_cap;
+    int n, n1, n2, l2, fr0, fr1, nblk, nav, nblk_cap;
 };
 
 struct TrigArgs {
-    int n, n1, n2, nfftf, fr0, fr1;
+    int n, n1, n2, l2, slot0, nfftf, fr0, fr1;
     double thr;
     int fstat_norm, nblk, nav, nblk_cap, numlines, cap;
     const double *lines;
-    const TmplDesc *desc;
+    SkyScal sk;
 };
 
 struct TrigEntry {
@@ -124,5 +154,8 @@ hipError_t launch_fstat_blocks(const void *FT, void *mu, const FstatArgs &a, int
 template <typename T>
 hipError_t launch_triggers(const void *FT, const void *mu, const TrigArgs &a, int nslots, TrigEntry *out,
                            int *counters, hipStream_t st);
+template <typename T>
+hipError_t launch_candidates(const void *FT, const TrigArgs &a, const TrigEntry *cand, const int *cand_cnt,
+                             int cand_cap, TrigEntry *out, int *counters, hipStream_t st);
 
 }  // namespace tdf

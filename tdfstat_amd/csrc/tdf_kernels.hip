@@ -25,6 +25,21 @@
 namespace tdf {
 
 // ---------------------------------------------------------------------
+// n.DetSSB[i] and i^2 + 2i*shft with separate multiply and add roundings
+// (no FMA contraction), as in the reference loop (jobcore.c:287-299).  shft
+// at i = 0 then equals shft1 exactly and shftf[0] = 0, which keeps the
+// spline abscissa 2(i - shftf[i]) at i = 0 inside [0, n-1].
+__device__ __forceinline__ double shft_dot(double n0, double n1, double n2, const double *v)
+{
+#pragma clang fp contract(off)
+    return n0 * v[0] + n1 * v[1] + n2 * v[2];
+}
+__device__ __forceinline__ double tmp1_of(int i, double shft)
+{
+#pragma clang fp contract(off)
+    return (double)i * (double)i + (double)(2 * i) * shft;
+}
+
 // Per-sky prep: one thread per sample i < nfft of detector blockIdx.y.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restrict__ xab)
@@ -37,10 +52,10 @@ __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restri
         cx<T> xa = {(T)0, (T)0}, xb = {(T)0, (T)0};
         if (i < a.N) {
             const float x = a.xDat[(size_t)d * a.N + i];
-            const double *ssb = a.DetSSB + (size_t)d * 3 * a.N + 3 * (size_t)i;
-            const double shft = a.ns0 * ssb[0] + a.ns1 * ssb[1] + a.ns2 * ssb[2];
-            const double shftf = shft - a.shft1[d];
-            const double tmp1 = (double)i * (double)i + (double)(2 * i) * shft;
+            const double *ssb0 = a.DetSSB + (size_t)d * 3 * a.N;
+            const double shft = shft_dot(a.ns0, a.ns1, a.ns2, ssb0 + 3 * (size_t)i);
+            const double shftf = shft - shft_dot(a.ns0, a.ns1, a.ns2, ssb0);
+            const double tmp1 = tmp1_of(i, shft);
             a.shftf[(size_t)d * a.N + i] = shftf;
             a.tmp1[(size_t)d * a.N + i] = tmp1;
             double aa = 0., bb = 0.;
@@ -134,6 +149,16 @@ __global__ void __launch_bounds__(FftCfg<T>::NT) k_fft_pass(IO io, const cx<T> *
     io.template store<SIGN, L, NCOL>(tile);
 }
 
+// Column tile of a pass-1 workgroup.  Workgroups are dispatched to the 8
+// XCDs round-robin by linear id; mapping consecutive ids to tiles
+// (id % 8) * (ntiles / 8) + id / 8 gives every XCD a contiguous range of
+// columns, so the cache lines of neighbouring tiles (and the inputs that all
+// templates of the batch re-read) stay in one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int bx, int nt)
+{
+    return (nt & 7) ? bx : (bx & 7) * (nt >> 3) + (bx >> 3);
+}
+
 // W_n^e from the two-level table (e < n), conjugated for SIGN>0
 template <int SIGN, typename T>
 __device__ __forceinline__ cx<T> twiddle_n(const cx<T> *__restrict__ lo, const cx<T> *__restrict__ hi,
@@ -158,7 +183,7 @@ struct IO_P1Nat {
     {
         constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)(2 * blockIdx.y) * n;
-        const int c0 = blockIdx.x * TC;
+        const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int cl = e % TC, q = e / TC;
             const int i = c0 + cl + n1 * q;
@@ -176,7 +201,7 @@ struct IO_P1Nat {
     {
         constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)(2 * blockIdx.y) * n;
-        const int c0 = blockIdx.x * TC;
+        const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int cl = e % TC, k2 = e / TC;
             const int c = c0 + cl;
@@ -230,7 +255,7 @@ template <typename T>
 struct IO_SpinP1 {
     const cx<T> *xm;       // [nifo][2][N]
     const double *tmp1;    // [nifo][N]
-    const TmplDesc *desc;  // [batch]
+    SkyScal sk;
     cx<T> *Z;              // [batch][2][n]
     const cx<T> *tw_lo, *tw_hi;
     int N, n, n1, nifo, slot0;
@@ -238,8 +263,8 @@ struct IO_SpinP1 {
     __device__ __forceinline__ void load(cx<T> *tile)
     {
         constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
-        const TmplDesc td = desc[slot0 + blockIdx.y];
-        const int c0 = blockIdx.x * TC;
+        const TmplDesc td = tmpl_scalars(sk, sk.iss0 + slot0 + blockIdx.y);
+        const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int cl = e % TC, q = e / TC;
             const int i = c0 + cl + n1 * q;
@@ -267,25 +292,32 @@ struct IO_SpinP1 {
     {
         constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)blockIdx.y * 2 * n;
-        const int c0 = blockIdx.x * TC;
+        const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int cl = e % TC, k2 = e / TC;
             const int c = c0 + cl;
             const cx<T> w = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * k2));
-            Z[base + (size_t)k2 * n1 + c] = cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w);
-            Z[base + n + (size_t)k2 * n1 + c] = cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w);
+            st_stream(&Z[base + (size_t)k2 * n1 + c], cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w));
+            st_stream(&Z[base + n + (size_t)k2 * n1 + c], cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w));
         }
     }
 };
 
 // Spindown pass 2: store = F-statistic of bins fr0..fr1 into
-// FT[slot][k2*n1 + k1] (bin k = k2 + n2*k1).
+// FT[slot][k2*n1 + k1] (bin k = k2 + n2*k1).  Without FStat normalisation
+// every bin of the MAX_ALG-3 candidate range with F >= thr is also appended
+// to a candidate list (wavefront ballot, one atomic per wave).
 template <typename T>
 struct IO_SpinP2 {
     const cx<T> *Z;
     T *FT;
     const double *sky;  // aa, bb
     int n, n1, n2, fr0, fr1;
+    int emit_cand, slot0;
+    double thr;
+    TrigEntry *cand;
+    int *cand_cnt;
+    int cand_cap;
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
@@ -295,8 +327,8 @@ struct IO_SpinP2 {
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int q = e % L, cl = e / L;
             const size_t off = (size_t)(k20 + cl) * n1 + q;
-            tile[cl * LP + ldsgeom<L>::pad(q)] = Z[base + off];
-            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = Z[base + n + off];
+            tile[cl * LP + ldsgeom<L>::pad(q)] = ld_stream(&Z[base + off]);
+            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = ld_stream(&Z[base + n + off]);
         }
     }
     template <int SIGN, int L, int NCOL>
@@ -306,14 +338,32 @@ struct IO_SpinP2 {
         const size_t base = (size_t)blockIdx.y * n;
         const int k20 = blockIdx.x * TC;
         const T aa = (T)sky[0], bb = (T)sky[1];
+        const T thrT = (T)thr;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int k1 = e % L, cl = e / L;
             const int k2 = k20 + cl;
             const int k = k2 + n2 * k1;
+            bool c = false;
+            T F = 0;
             if (k >= fr0 && k <= fr1) {
-                const T F = cnorm(tile[cl * LP + ldsgeom<L>::pad(k1)]) / aa +
-                            cnorm(tile[(TC + cl) * LP + ldsgeom<L>::pad(k1)]) / bb;
+                F = cnorm(tile[cl * LP + ldsgeom<L>::pad(k1)]) / aa +
+                    cnorm(tile[(TC + cl) * LP + ldsgeom<L>::pad(k1)]) / bb;
                 FT[base + (size_t)k2 * n1 + k1] = F;
+                c = emit_cand && k > fr0 && k < fr1 - 1 && !(F < thrT);
+            }
+            if (emit_cand) {
+                const unsigned long long m = __ballot(c);
+                if (m) {
+                    const int lane = threadIdx.x & 63;
+                    const int leader = __ffsll((long long)m) - 1;
+                    int bpos = 0;
+                    if (lane == leader) bpos = atomicAdd(cand_cnt, __popcll(m));
+                    bpos = __shfl(bpos, leader, 64);
+                    if (c) {
+                        const int pos = bpos + __popcll(m & ((1ull << lane) - 1ull));
+                        if (pos < cand_cap) cand[pos] = TrigEntry{slot0 + (int)blockIdx.y, k, 0.f, (float)F};
+                    }
+                }
             }
         }
     }
@@ -447,21 +497,28 @@ __global__ void __launch_bounds__(256) k_splint(const cx<T> *__restrict__ Y, con
 }
 
 // ---------------------------------------------------------------------
-// F-statistic tiles: natural bins [s, s+TB) of one spindown slot, read
-// from the column-major FT with lanes along k1 (contiguous in FT).
+// F-statistic tiles: natural bins [lo, hi) of one spindown slot, read from
+// the column-major FT (row k2 holds bins k2 + n2*k1) one row per wavefront
+// with lanes along k1 (contiguous in FT).  Bin k is stored at LDS index
+// fpad(k - lo): one pad slot per n2 bins keeps the stride-n2 writes of a row
+// on distinct banks.
+__device__ __forceinline__ int fpad(int j, int l2) { return j + (j >> l2); }
+
 template <typename T>
-__device__ __forceinline__ void load_f_tile(const T *__restrict__ FT, T *nat, int s, int lo, int hi,
-                                            int fr0, int fr1, int n1, int n2)
+__device__ __forceinline__ void load_f_tile(const T *__restrict__ FT, T *nat, int lo, int hi, int fr0, int fr1,
+                                            int n1, int n2, int l2)
 {
-    // nat[k - lo] = F[k] for k in [lo, hi) (0 outside [fr0, fr1])
-    const int k1a = (lo < 0 ? 0 : lo) / n2;
-    const int k1b = (hi - 1) / n2;
+    const int k1a = (lo < 0 ? 0 : lo) >> l2;
+    const int k1b = (hi - 1) >> l2;
     const int dk = k1b - k1a + 1;
-    const int tot = dk * n2;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int k1 = k1a + e % dk, k2 = e / dk;
-        const int k = k2 + n2 * k1;
-        if (k >= lo && k < hi) nat[k - lo] = (k >= fr0 && k <= fr1) ? FT[(size_t)k2 * n1 + k1] : (T)0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (int k2 = wave; k2 < n2; k2 += nwave) {
+        const T *row = FT + (size_t)k2 * n1;
+        for (int u = lane; u < dk; u += 64) {
+            const int k1 = k1a + u;
+            const int k = k2 + (k1 << l2);
+            if (k >= lo && k < hi) nat[fpad(k - lo, l2)] = (k >= fr0 && k <= fr1) ? row[k1] : (T)0;
+        }
     }
 }
 
@@ -472,7 +529,7 @@ __global__ void __launch_bounds__(256) k_fstat_blocks(const T *__restrict__ FT, 
                                                       FstatArgs a)
 {
     constexpr int TRIG_TB = TrigCfg<T>::TB;
-    __shared__ T nat[TRIG_TB];
+    __shared__ T nat[TRIG_TB + TRIG_TB / 32 + 2];
     __shared__ T cs[256];
     const int slot = blockIdx.y;
     const T *ft = FT + (size_t)slot * a.n;
@@ -480,7 +537,7 @@ __global__ void __launch_bounds__(256) k_fstat_blocks(const T *__restrict__ FT, 
     const int end_norm = a.fr0 + a.nblk * a.nav;
     const int hi = min(s + TRIG_TB, end_norm);
     if (s >= hi) return;
-    load_f_tile(ft, nat, s, s, hi, a.fr0, a.fr1, a.n1, a.n2);
+    load_f_tile(ft, nat, s, hi, a.fr0, a.fr1, a.n1, a.n2, a.l2);
     __syncthreads();
     constexpr int CH = TRIG_TB / 256;
     const int nvalid = hi - s;
@@ -490,7 +547,7 @@ __global__ void __launch_bounds__(256) k_fstat_blocks(const T *__restrict__ FT, 
         const int b0 = threadIdx.x * CH;
         for (int u = 0; u < CH; u++) {
             const int b = b0 + u;
-            if (b < nvalid) acc += nat[b];
+            if (b < nvalid) acc += nat[fpad(b, a.l2)];
         }
         cs[threadIdx.x] = acc;
         __syncthreads();
@@ -508,7 +565,7 @@ __global__ void __launch_bounds__(256) k_fstat_blocks(const T *__restrict__ FT, 
         const int nb = nvalid / a.nav;
         for (int bl = threadIdx.x; bl < nb; bl += 256) {
             T acc = 0;
-            for (int u = 0; u < a.nav; u++) acc += nat[bl * a.nav + u];
+            for (int u = 0; u < a.nav; u++) acc += nat[fpad(bl * a.nav + u, a.l2)];
             const int blk = (s - a.fr0) / a.nav + bl;
             mu[(size_t)slot * a.nblk_cap + blk] = (T)((double)acc / (2. * a.nav));
         }
@@ -519,7 +576,7 @@ template <typename T>
 __device__ __forceinline__ T fnorm_at(const T *__restrict__ ft, const T *__restrict__ mu, const TrigArgs &a,
                                       int k)
 {
-    const int k2 = k % a.n2, k1 = k / a.n2;
+    const int k2 = k & (a.n2 - 1), k1 = k >> a.l2;
     T f = ft[(size_t)k2 * a.n1 + k1];
     if (a.fstat_norm && k < a.fr0 + a.nblk * a.nav) f = f / mu[(k - a.fr0) / a.nav];
     return f;
@@ -542,21 +599,23 @@ __global__ void __launch_bounds__(256) k_triggers(const T *__restrict__ FT, cons
                                                   int *__restrict__ counters)
 {
     constexpr int TRIG_TB = TrigCfg<T>::TB;
-    __shared__ T nat[TRIG_TB + 2];
+    __shared__ T nat[TRIG_TB + TRIG_TB / 32 + 4];
     __shared__ unsigned char cf[TRIG_TB];
-    const int slot = blockIdx.y;
+    const int slot = blockIdx.y;            // slot within this launch
+    const int gslot = a.slot0 + slot;       // template index of the call
     const T *ft = FT + (size_t)slot * a.n;
     const T *mu = MU + (size_t)slot * a.nblk_cap;
     const int s = a.fr0 + blockIdx.x * TRIG_TB;
     const int e_end = min(s + TRIG_TB, a.fr1 + 1);  // bins of this tile
     if (s >= e_end) return;
-    load_f_tile(ft, nat, s, s - 1, e_end + 1, a.fr0, a.fr1, a.n1, a.n2);
+    const int l2 = a.l2;
+    load_f_tile(ft, nat, s - 1, e_end + 1, a.fr0, a.fr1, a.n1, a.n2, l2);
     __syncthreads();
     const int end_norm = a.fr0 + a.nblk * a.nav;
     if (a.fstat_norm) {
         for (int e = threadIdx.x; e < e_end + 1 - (s - 1); e += 256) {
             const int k = s - 1 + e;
-            if (k >= a.fr0 && k < end_norm) nat[e] = nat[e] / mu[(k - a.fr0) / a.nav];
+            if (k >= a.fr0 && k < end_norm) nat[fpad(e, l2)] = nat[fpad(e, l2)] / mu[(k - a.fr0) / a.nav];
         }
         __syncthreads();
     }
@@ -565,13 +624,13 @@ __global__ void __launch_bounds__(256) k_triggers(const T *__restrict__ FT, cons
         const int i = s + e;
         bool c = false;
         if (i >= ilo && i <= ihi && i < e_end) {
-            const T fi = nat[e + 1];
-            c = !(fi < (T)a.thr || nat[e] > fi || fi < nat[e + 2]);
+            const T fi = nat[fpad(e + 1, l2)];
+            c = !(fi < (T)a.thr || nat[fpad(e, l2)] > fi || fi < nat[fpad(e + 2, l2)]);
         }
         cf[e] = c;
     }
     __syncthreads();
-    const double sgnl0 = a.desc[slot].sgnl0;
+    const double sgnl0 = tmpl_scalars(a.sk, a.sk.iss0 + gslot).sgnl0;
     for (int e0 = 0; e0 < TRIG_TB; e0 += 256) {
         const int e = e0 + threadIdx.x;
         const int i = s + e;
@@ -603,7 +662,7 @@ __global__ void __launch_bounds__(256) k_triggers(const T *__restrict__ FT, cons
                 if (!veto) {
                     emit = true;
                     fo = (float)f_;
-                    Fo = (float)nat[e + 1];
+                    Fo = (float)nat[fpad(e + 1, l2)];
                 }
             }
         }
@@ -615,13 +674,72 @@ __global__ void __launch_bounds__(256) k_triggers(const T *__restrict__ FT, cons
             int base = 0;
             if (lane == leader) {
                 base = atomicAdd(&counters[0], __popcll(m));
-                atomicAdd(&counters[1 + slot], __popcll(m));
+                atomicAdd(&counters[1 + gslot], __popcll(m));
             }
             base = __shfl(base, leader, 64);
             if (emit) {
                 const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-                if (pos < a.cap) out[pos] = TrigEntry{slot, i, fo, Fo};
+                if (pos < a.cap) out[pos] = TrigEntry{gslot, i, fo, Fo};
             }
+        }
+    }
+}
+
+// Candidates (bins with F >= thr from pass 2, no FStat normalisation):
+// local-maximum test against the neighbouring bins in FT, the sequential
+// skip rule (run parity), veto lines and compaction as in k_triggers.
+template <typename T>
+__global__ void __launch_bounds__(256) k_candidates(const T *__restrict__ FT, TrigArgs a,
+                                                    const TrigEntry *__restrict__ cand,
+                                                    const int *__restrict__ cand_cnt, int cand_cap,
+                                                    TrigEntry *__restrict__ out, int *__restrict__ counters)
+{
+    const int ncand = min(*cand_cnt, cand_cap);
+    const int ilo = a.fr0 + 1;
+    const int nloop = (ncand + gridDim.x * 256 - 1) / (gridDim.x * 256);
+    for (int it = 0; it < nloop; it++) {
+        const int idx = (it * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
+        bool emit = false;
+        float fo = 0.f, Fo = 0.f;
+        int gslot = 0, i = 0;
+        if (idx < ncand) {
+            const TrigEntry ce = cand[idx];
+            gslot = ce.slot;
+            i = ce.ic;
+            const T *ft = FT + (size_t)(gslot - a.slot0) * a.n;
+            if (cond_global(ft, (const T *)nullptr, a, i)) {
+                int run = 0;
+                int ig = i - 1;
+                while (ig >= ilo && cond_global(ft, (const T *)nullptr, a, ig)) {
+                    run++;
+                    ig--;
+                }
+                if ((run & 1) == 0) {
+                    const double sgnl0 = tmpl_scalars(a.sk, a.sk.iss0 + gslot).sgnl0;
+                    const double f_ = (double)(2 * i) / (double)a.nfftf * 3.14159265358979323846 + sgnl0;
+                    bool veto = false;
+                    for (int l = 0; l < a.numlines; l++)
+                        if (f_ >= a.lines[2 * l] && f_ <= a.lines[2 * l + 1]) {
+                            veto = true;
+                            break;
+                        }
+                    if (!veto) {
+                        emit = true;
+                        fo = (float)f_;
+                        Fo = (float)fnorm_at(ft, (const T *)nullptr, a, i);
+                    }
+                }
+            }
+        }
+        const unsigned long long m = __ballot(emit);
+        if (m) {
+            const int lane = threadIdx.x & 63;
+            if (emit) {
+                const int pos = atomicAdd(&counters[0], 1);
+                atomicAdd(&counters[1 + gslot], 1);
+                if (pos < a.cap) out[pos] = TrigEntry{gslot, i, fo, Fo};
+            }
+            (void)lane;
         }
     }
 }
@@ -720,7 +838,7 @@ hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, con
 template <typename T>
 hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
-    IO_SpinP1<T> io{(const cx<T> *)s.xm, s.tmp1, s.desc, (cx<T> *)s.Z, (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D],
+    IO_SpinP1<T> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z, (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D],
                     (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D], s.N, p.n, p.n1, s.nifo, s.slot0};
     const int tc1 = FftCfg<T>::ELEMS / p.n2 / 2;
     return launch_pass<T, -1>(p.n2, io, dim3(p.n1 / tc1, s.nslots),
@@ -730,7 +848,8 @@ hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st
 template <typename T>
 hipError_t launch_spin_pass2(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
-    IO_SpinP2<T> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1};
+    IO_SpinP2<T> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1,
+                    s.emit_cand, s.slot0, s.thr, s.cand, s.cand_cnt, s.cand_cap};
     const int tc2 = FftCfg<T>::ELEMS / p.n1 / 2;
     return launch_pass<T, -1>(p.n1, io, dim3(p.n2 / tc2, s.nslots),
                               (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1], st);
@@ -758,6 +877,15 @@ hipError_t launch_triggers(const void *FT, const void *mu, const TrigArgs &a, in
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_candidates(const void *FT, const TrigArgs &a, const TrigEntry *cand, const int *cand_cnt,
+                             int cand_cap, TrigEntry *out, int *counters, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_candidates<T>, dim3(64), dim3(256), 0, st, (const T *)FT, a, cand, cand_cnt, cand_cap,
+                       out, counters);
+    return hipGetLastError();
+}
+
 #define TDF_INST(T)                                                                                        \
     template hipError_t launch_sky_prep<T>(const SkyPrepArgs &, void *, int, int, hipStream_t);           \
     template hipError_t launch_fft_natural_to_T<T>(const FftPlan &, const void *, void *, void *, int, int, \
@@ -772,7 +900,9 @@ hipError_t launch_triggers(const void *FT, const void *mu, const TrigArgs &a, in
     template hipError_t launch_spin_pass2<T>(const FftPlan &, const SpinArgs &, hipStream_t);              \
     template hipError_t launch_fstat_blocks<T>(const void *, void *, const FstatArgs &, int, hipStream_t);  \
     template hipError_t launch_triggers<T>(const void *, const void *, const TrigArgs &, int, TrigEntry *,  \
-                                           int *, hipStream_t);
+                                           int *, hipStream_t);                                            \
+    template hipError_t launch_candidates<T>(const void *, const TrigArgs &, const TrigEntry *, const int *, \
+                                             int, TrigEntry *, int *, hipStream_t);
 
 TDF_INST(float)
 TDF_INST(double)

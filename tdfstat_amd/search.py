@@ -87,8 +87,10 @@ class Context:
         self.sett = sett
         self.prec = prec
         self._g = geom_from(sett)
+        # spindowns per launch: enough workgroups to fill 256 CUs several times
+        self.batch = int(batch) if batch else (64 if sett.nfftf >= (1 << 19) else 128)
         h = C.c_void_p()
-        L.check(self.lib.tdf_ctx_create(C.byref(self._g), prec, batch, C.byref(h)), "tdf_ctx_create")
+        L.check(self.lib.tdf_ctx_create(C.byref(self._g), prec, self.batch, C.byref(h)), "tdf_ctx_create")
         self.h = h
         for k, d in enumerate(dets):
             x = np.ascontiguousarray(d.xDat, dtype=np.float32)
@@ -148,6 +150,22 @@ class Context:
                 "tdf_job_core")
         return sky, _records(recs, nrec.value, pairs)
 
+    def job_core_raw(self, pm, mm, nn, opts_c, max_spindowns, pair_cap):
+        """job_core into buffers owned by the context; returns
+        (sky, nrec, npairs).  opts_c comes from JobOpts.to_c()."""
+        key = (max_spindowns, pair_cap)
+        if getattr(self, "_rawkey", None) != key:
+            self._raw_recs = (L.TdfRecord * max_spindowns)()
+            self._raw_pairs = np.zeros(2 * pair_cap, dtype=np.float32)
+            self._rawkey = key
+        sky = L.TdfSky()
+        nrec = C.c_int(0)
+        npairs = C.c_longlong(0)
+        L.check(self.lib.tdf_job_core(self.h, pm, mm, nn, C.byref(opts_c), C.byref(sky), self._raw_recs,
+                                      max_spindowns, L.fptr(self._raw_pairs), pair_cap, C.byref(nrec),
+                                      C.byref(npairs)), "tdf_job_core")
+        return sky, nrec.value, npairs.value
+
     # -- inspection --
     def sky_arrays(self, det):
         N = self.sett.N
@@ -156,6 +174,12 @@ class Context:
         L.check(self.lib.tdf_debug_sky_arrays(self.h, det, L.dptr(tmp1), L.dptr(shftf), L.dptr(xa),
                                               L.dptr(xb)), "tdf_debug_sky_arrays")
         return tmp1, shftf, xa.view(np.complex128), xb.view(np.complex128)
+
+    def sky_interp(self, item):
+        n = self.sett.interpftpad * self.sett.N
+        y, y2 = np.zeros(2 * n), np.zeros(2 * n)
+        L.check(self.lib.tdf_debug_sky_interp(self.h, item, L.dptr(y), L.dptr(y2)), "tdf_debug_sky_interp")
+        return y.view(np.complex128), y2.view(np.complex128)
 
     def fstat(self, iss):
         F = np.zeros(self.sett.nfftf)
