@@ -94,12 +94,15 @@ def kernel_bytes(sett, S, nslots, kname, prec64):
     return None
 
 
-def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0):
+def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True):
     from tdfstat_amd import search as TS
     sett = seg.sett
     S = cfg["nspin"]
     opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
     ctx = TS.Context(sett, seg.dets, prec=prec, batch=batch)
+    if tune:
+        plan = ctx.autotune()
+        print(f"[bench] prec={prec} autotuned plan {plan}", file=sys.stderr, flush=True)
     oc = opts.to_c()
     cap = (S + 1) * 64
     for k in range(warmup):
@@ -125,23 +128,28 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0):
 
 def fp32_vs_fp64_error(cfg, seg, pt, opts, ctx32, ctx64, nss=8):
     """max / mean relative F error of the FP32 path vs the FP64 path over
-    bins with F64 >= 2, and trigger-set equality, on nss spindowns."""
+    bins with F64 >= 2 on nss spindowns, and trigger-set equality over the
+    full spindown range of sky point pt."""
     S = cfg["nspin"]
     sky32, r32 = ctx32.job_core(*pt, opts, max_spindowns=S + 1)
     sky64, r64 = ctx64.job_core(*pt, opts, max_spindowns=S + 1)
-    errs = []
-    last0 = (S - 1) - ((S - 1) % ctx32.batch)
-    for iss in range(last0, min(S, last0 + nss)):
-        a, b = ctx32.fstat(iss), ctx64.fstat(iss)
-        sel = b >= 2.0
-        errs.append(np.abs(a[sel] - b[sel]) / b[sel])
-    e = np.concatenate(errs) if errs else np.zeros(1)
     thr = opts.thr
     same = True
     for x, y in zip(r32, r64):
         fx = {round(float(f), 6) for f, v in x.ffstat if abs(v - thr) > 1e-3 * thr}
         fy = {round(float(f), 6) for f, v in y.ffstat if abs(v - thr) > 1e-3 * thr}
         same &= fx == fy
+    # F vectors of the same nss spindowns (one launch each, so both are the
+    # last batch of their context)
+    i0 = S // 2 - nss // 2
+    ctx32.spindown(i0, nss, opts)
+    ctx64.spindown(i0, nss, opts)
+    errs = []
+    for iss in range(i0, i0 + nss):
+        a, b = ctx32.fstat(iss), ctx64.fstat(iss)
+        sel = b >= 2.0
+        errs.append(np.abs(a[sel] - b[sel]) / b[sel])
+    e = np.concatenate(errs) if errs else np.zeros(1)
     return float(e.max()), float(e.mean()), bool(same)
 
 
@@ -183,7 +191,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--batch", type=int, default=0, help="spindowns per launch (0 = library default)")
+    ap.add_argument("--batch", type=int, default=0, help="max spindowns per launch (0 = library default)")
+    ap.add_argument("--no-tune", action="store_true", help="use the default plan instead of tdf_autotune")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -211,11 +220,12 @@ def main():
           f"sky={len(pts)} S={S}", file=sys.stderr, flush=True)
 
     ctx32, opts, nt, el, kt, ntrig = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
-                                             args.batch)
+                                             args.batch, not args.no_tune)
     value = nt / el
     print(f"[bench] fp32: {nt} templates in {el:.3f} s -> {value:.0f} templates/s", file=sys.stderr, flush=True)
     k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 4)
-    ctx64, _, nt64, el64, kt64, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch)
+    ctx64, _, nt64, el64, kt64, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
+                                            not args.no_tune)
     value64 = nt64 / el64
     print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
           flush=True)
@@ -226,6 +236,7 @@ def main():
     dom = max(spin, key=lambda k: spin[k][0])
     ms_tot, launches = spin[dom]
     nslots_avg = nt / max(1, kt["spin_pass1_demod_fft"][1])
+    line_plan = ctx32.plan_info()
     kb = kernel_bytes(sett, S, nslots_avg, dom, False)
     ach = kb / (ms_tot / launches * 1e-3) / 1e9
     bt = algorithmic_bytes(sett, S, False)
@@ -246,7 +257,7 @@ def main():
         "config": {"workload": f"{args.config}: nifo={sett.nifo} N={sett.N} nfft={sett.nfft} "
                                f"nfftf={sett.nfftf} {len(pts)} sky x {S} spindowns, seeded Gaussian "
                                f"noise + injections, FP64 phase",
-                   "templates_timed": nt, "triggers": ntrig},
+                   "templates_timed": nt, "triggers": ntrig, "plan": line_plan},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                      "avg_launch_ms": round(ms_tot / launches, 5),

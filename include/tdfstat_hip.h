@@ -158,6 +158,25 @@ int tdf_debug_fstat(tdf_ctx *ctx, int iss, double *F);
 int tdf_fft_c2c(const double *in, double *out, int n, int howmany, int sign,
                 int prec);
 
+/* sin and cos of n FP64 phases as the FP32 path evaluates them (mode 0:
+ * the kernels' path, FP64 reduction to a fraction of a turn + hardware
+ * sin/cos; mode 1: FP64 reduction modulo pi/2 + FP32 polynomials). */
+int tdf_sincos_probe(const double *ph, int n, int mode, float *s, float *c);
+
+/* ---- tuning ---- */
+/* Times the spindown kernels for every supported four-step factorisation
+ * n1 x n2 of nfftf, LDS tile size of both passes and spindowns per launch
+ * (up to the batch given at creation) and keeps the fastest.  Without this
+ * call the context uses a fixed default plan (results then do not depend
+ * on timing). */
+int tdf_autotune(tdf_ctx *ctx);
+/* the plan in use: pass-2 length n1 (nfftf = n1 * n2), LDS tile elements
+ * e1/e2 of the two passes, fused = pass 2 reads Z and writes F directly from
+ * registers, batch = spindowns per launch */
+int tdf_plan_info(tdf_ctx *ctx, int *n1, int *e1, int *e2, int *fused, int *batch);
+/* selects a plan explicitly (same meaning; batch <= the creation batch) */
+int tdf_set_plan(tdf_ctx *ctx, int n1, int e1, int e2, int fused, int batch);
+
 /* ---- timing ---- */
 /* When enabled, every kernel launch of the spindown loop is bracketed
  * by HIP events on the context's stream; tdf_kernel_times returns, per

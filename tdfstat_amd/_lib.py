@@ -75,6 +75,11 @@ SIGNATURES = {
     "tdf_debug_fstat": (C.c_int, [_P, C.c_int, _DP]),
     "tdf_debug_sky_interp": (C.c_int, [_P, C.c_int, _DP, _DP]),
     "tdf_fft_c2c": (C.c_int, [_DP, _DP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "tdf_sincos_probe": (C.c_int, [_DP, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "tdf_autotune": (C.c_int, [_P]),
+    "tdf_plan_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "tdf_set_plan": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "tdf_profile_enable": (C.c_int, [_P, C.c_int]),
     "tdf_kernel_times": (C.c_int, [_P, C.c_int, _DP, C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_char_p)]),

@@ -64,6 +64,13 @@ struct tdf_ctx {
     std::vector<void *> allocs;
     const void *twL[2][16] = {};
     FftPlan p_fwd{}, p_inv{}, p_spin{};
+    struct TwN {
+        int n;
+        const void *lo, *hi;
+    };
+    std::vector<TwN> twn;  // two-level W_n tables by n
+    int batch_max = 0;     // allocated spindowns per launch
+    bool tuned = false;
     // detector data
     float *xDat = nullptr;
     double *DetSSB = nullptr;
@@ -192,22 +199,36 @@ int make_table(tdf_ctx *c, long long L, long long count, long long stride, const
     return TDF_OK;
 }
 
-int make_plan(tdf_ctx *c, int n, FftPlan *p)
+// Four-step factorisation n = n1*n2 with the given pass-2 length n1 (0 =
+// default: the longest pass 2 that fits one tile) and tile sizes e1/e2
+// (0 = largest).
+int make_plan(tdf_ctx *c, int n, FftPlan *p, int n1_req = 0, int e1 = 0, int e2 = 0)
 {
     if (!is_pow2(n) || n < 4096) return TDF_ERR_UNSUPPORTED;
-    const int elems_half = c->prec == TDF_PREC_FP64 ? 1024 : 2048;
-    int n1 = std::min(elems_half, n / 64);
+    const int emax = c->prec == TDF_PREC_FP64 ? 2048 : 4096;
+    const int elems_half = emax / 2;
+    int n1 = n1_req > 0 ? n1_req : std::min(elems_half, n / 64);
+    if (!is_pow2(n1) || n1 < 64 || n1 > elems_half || n % n1) return TDF_ERR_UNSUPPORTED;
     int n2 = n / n1;
-    if (n2 > elems_half) return TDF_ERR_UNSUPPORTED;
+    if (n2 > elems_half || n2 < 64) return TDF_ERR_UNSUPPORTED;
     p->n = n;
     p->n1 = n1;
     p->n2 = n2;
     p->l1 = ilog2i(n1);
     p->l2 = ilog2i(n2);
+    p->e1 = e1 > 0 ? e1 : emax;
+    p->e2 = e2 > 0 ? e2 : emax;
+    if (p->e1 < 2 * n2 || p->e2 < 2 * n1 || p->e1 > emax || p->e2 > emax) return TDF_ERR_UNSUPPORTED;
     for (int d = 0; d < 2; d++)
         for (int l = 0; l < 16; l++) p->twL[d][l] = c->twL[d][l];
     int rc;
     const int dd = c->prec == TDF_PREC_FP64 ? 1 : 0;
+    for (auto &t : c->twn)
+        if (t.n == n) {
+            p->tw_lo[dd] = t.lo;
+            p->tw_hi[dd] = t.hi;
+            return TDF_OK;
+        }
     if (dd) {
         if ((rc = make_table<double>(c, n, TW_LO, 1, &p->tw_lo[1]))) return rc;
         if ((rc = make_table<double>(c, n, std::max(1, n / TW_LO), TW_LO, &p->tw_hi[1]))) return rc;
@@ -215,8 +236,11 @@ int make_plan(tdf_ctx *c, int n, FftPlan *p)
         if ((rc = make_table<float>(c, n, TW_LO, 1, &p->tw_lo[0]))) return rc;
         if ((rc = make_table<float>(c, n, std::max(1, n / TW_LO), TW_LO, &p->tw_hi[0]))) return rc;
     }
+    c->twn.push_back({n, p->tw_lo[dd], p->tw_hi[dd]});
     return TDF_OK;
 }
+
+SpinLayout spin_layout(const tdf_ctx *c) { return SpinLayout{c->p_spin.n1, c->p_spin.l1, c->p_spin.n2}; }
 
 template <typename T>
 int run_sky_stage(tdf_ctx *c, const SkyPrepArgs &a)
@@ -265,7 +289,8 @@ int run_sky_stage(tdf_ctx *c, const SkyPrepArgs &a)
     {
         Prof pr(c, KS_SPLINT);
         HIPCHK(launch_splint<T>(c->Ymv, c->Zsky, c->shftf, c->sig2_d, c->xm, g.N, nnodes, g.interpftpad,
-                                (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, st));
+                                (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, spin_layout(c),
+                                (size_t)g.nfftf, st));
     }
     return TDF_OK;
 }
@@ -470,8 +495,8 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
     TRY(dmalloc(c, &c->XT, cs * 2 * nifo * (size_t)g->Ninterp));
     TRY(dmalloc(c, &c->Ymv, cs * 2 * nifo * (size_t)g->Ninterp));
     TRY(dmalloc(c, (void **)&c->shftf, sizeof(double) * nifo * (size_t)N));
-    TRY(dmalloc(c, (void **)&c->tmp1, sizeof(double) * nifo * (size_t)N));
-    TRY(dmalloc(c, &c->xm, cs * 2 * nifo * (size_t)N));
+    TRY(dmalloc(c, (void **)&c->tmp1, sizeof(double) * nifo * (size_t)g->nfftf));
+    TRY(dmalloc(c, &c->xm, cs * 2 * nifo * (size_t)g->nfftf));
     c->prep_nblk = (g->nfft + 255) / 256;
     TRY(dmalloc(c, (void **)&c->partials, sizeof(double) * 2 * nifo * (size_t)c->prep_nblk));
     TRY(dmalloc(c, (void **)&c->sky_scal, sizeof(double) * 4));
@@ -490,6 +515,10 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
             TRY(TDF_ERR_HIP);
     }
     const size_t B = (size_t)c->batch;
+    c->batch_max = c->batch;
+    if (hipMemset(c->xm, 0, cs * 2 * nifo * (size_t)g->nfftf) != hipSuccess) TRY(TDF_ERR_HIP);
+    if (hipMemset(c->tmp1, 0, sizeof(double) * nifo * (size_t)g->nfftf) != hipSuccess) TRY(TDF_ERR_HIP);
+    if (hipMemset(c->sky_scal, 0, sizeof(double) * 4) != hipSuccess) TRY(TDF_ERR_HIP);
     TRY(dmalloc(c, &c->Z, cs * 2 * B * (size_t)g->nfftf));
     TRY(dmalloc(c, &c->FT, c->rsz * B * (size_t)g->nfftf));
     c->nblk_cap = g->nfftf / 16 + 1;  // nav >= 16 in practice; checked per call
@@ -560,6 +589,7 @@ int tdf_sky_stage(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o,
     s.ra = fmod(atan2(sinal, cosal) + 2. * M_PI, 2. * M_PI);
     s.het0 = fmod(nn * M[8] + mm * M[12], M[0]);
     SkyPrepArgs a{};
+    a.lay = spin_layout(c);
     a.xDat = c->xDat;
     a.DetSSB = c->DetSSB;
     a.det = c->detp;
@@ -748,20 +778,33 @@ int tdf_debug_sky_arrays(tdf_ctx *c, int det, double *tmp1, double *shftf, doubl
 {
     if (!c || det < 0 || det >= c->g.nifo) return TDF_ERR_ARG;
     if (!c->sky_valid) return TDF_ERR_STATE;
-    const size_t N = c->g.N;
+    const size_t N = c->g.N, nf = c->g.nfftf;
+    const SpinLayout lay = spin_layout(c);
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (tmp1) HIPCHK(hipMemcpy(tmp1, c->tmp1 + det * N, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (tmp1) {
+        std::vector<double> h(nf);
+        HIPCHK(hipMemcpy(h.data(), c->tmp1 + det * nf, sizeof(double) * nf, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) tmp1[i] = h[lay.at((int)i)];
+    }
     if (shftf) HIPCHK(hipMemcpy(shftf, c->shftf + det * N, sizeof(double) * N, hipMemcpyDeviceToHost));
     for (int w = 0; w < 2; w++) {
         double *dst = w == 0 ? xDatma : xDatmb;
         if (!dst) continue;
-        const char *src = (const char *)c->xm + (2 * det + w) * N * c->csz;
+        const char *src = (const char *)c->xm + (2 * det + w) * nf * c->csz;
         if (c->prec == TDF_PREC_FP64) {
-            HIPCHK(hipMemcpy(dst, src, 16 * N, hipMemcpyDeviceToHost));
+            std::vector<double> h(2 * nf);
+            HIPCHK(hipMemcpy(h.data(), src, 16 * nf, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; i++) {
+                dst[2 * i] = h[2 * lay.at((int)i)];
+                dst[2 * i + 1] = h[2 * lay.at((int)i) + 1];
+            }
         } else {
-            std::vector<float> h(2 * N);
-            HIPCHK(hipMemcpy(h.data(), src, 8 * N, hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < 2 * N; i++) dst[i] = h[i];
+            std::vector<float> h(2 * nf);
+            HIPCHK(hipMemcpy(h.data(), src, 8 * nf, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; i++) {
+                dst[2 * i] = h[2 * lay.at((int)i)];
+                dst[2 * i + 1] = h[2 * lay.at((int)i) + 1];
+            }
         }
     }
     return TDF_OK;
@@ -885,6 +928,127 @@ int tdf_fft_c2c(const double *in, double *out, int n, int howmany, int sign, int
         return TDF_ERR_HIP;
     }
     return TDF_OK;
+}
+
+int tdf_autotune(tdf_ctx *c)
+{
+    if (!c) return TDF_ERR_ARG;
+    const int n = c->g.nfftf;
+    const bool d = c->prec == TDF_PREC_FP64;
+    const int emax = d ? 2048 : 4096;
+    std::vector<int> n1s = {256, 512, 1024, 2048}, es = {512, 1024, 2048, 4096}, bs = {32, 64, 128, 256};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const bool prof = c->prof;
+    c->prof = false;
+    FftPlan best = c->p_spin;
+    int best_b = c->batch;
+    float best_t = 1e30f;
+    SkyScal sk{0.f, 1.f, 0.f, 0.f, 0.0, c->g.M[0], c->g.M[4], c->g.M[5], c->g.M[9], c->g.M[13], 0};
+    for (int n1 : n1s)
+        for (int ea : es)
+            for (int eb : es)
+                for (int f2 = 0; f2 < 2; f2++) {
+                if (ea > emax || eb > emax) continue;
+                FftPlan p{};
+                if (make_plan(c, n, &p, n1, ea, eb) != TDF_OK) continue;
+                if (f2 && (eb != 2 * n1 || n1 < 512 || (d && n1 > 1024))) continue;
+                p.f2 = f2;
+                for (int b : bs) {
+                    if (b > c->batch_max) continue;
+                    SpinArgs sa{};
+                    sa.xm = c->xm;
+                    sa.tmp1 = c->tmp1;
+                    sa.sk = sk;
+                    sa.Z = c->Z;
+                    sa.FT = c->FT;
+                    sa.sky = c->sky_scal;
+                    sa.N = c->g.N;
+                    sa.nifo = c->g.nifo;
+                    sa.nslots = b;
+                    sa.fr0 = 0;
+                    sa.fr1 = n / 2 - 1;
+                    float t = 0.f;
+                    for (int rep = 0; rep < 2; rep++) {
+                        HIPCHK(hipEventRecord(e0, c->stream));
+                        hipError_t er = d ? launch_spin_pass1<double>(p, sa, c->stream)
+                                          : launch_spin_pass1<float>(p, sa, c->stream);
+                        if (er == hipSuccess)
+                            er = d ? launch_spin_pass2<double>(p, sa, c->stream)
+                                   : launch_spin_pass2<float>(p, sa, c->stream);
+                        HIPCHK(er);
+                        HIPCHK(hipEventRecord(e1, c->stream));
+                        HIPCHK(hipEventSynchronize(e1));
+                        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+                    }
+                    const float per = t / b;
+                    if (per < best_t) {
+                        best_t = per;
+                        best = p;
+                        best_b = b;
+                    }
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->prof = prof;
+    if (best.n1 != c->p_spin.n1) c->sky_valid = false;  // per-sky inputs are laid out for n1
+    c->p_spin = best;
+    c->batch = best_b;
+    c->tuned = true;
+    return TDF_OK;
+}
+
+int tdf_plan_info(tdf_ctx *c, int *n1, int *e1, int *e2, int *fused, int *batch)
+{
+    if (!c) return TDF_ERR_ARG;
+    if (n1) *n1 = c->p_spin.n1;
+    if (e1) *e1 = c->p_spin.e1;
+    if (e2) *e2 = c->p_spin.e2;
+    if (fused) *fused = c->p_spin.f2;
+    if (batch) *batch = c->batch;
+    return TDF_OK;
+}
+
+int tdf_set_plan(tdf_ctx *c, int n1, int e1, int e2, int fused, int batch)
+{
+    if (!c || batch < 1 || batch > c->batch_max) return TDF_ERR_ARG;
+    const bool d = c->prec == TDF_PREC_FP64;
+    FftPlan p{};
+    int rc = make_plan(c, c->g.nfftf, &p, n1, e1, e2);
+    if (rc) return rc;
+    if (fused && (p.e2 != 2 * p.n1 || p.n1 < 512 || (d && p.n1 > 1024))) return TDF_ERR_UNSUPPORTED;
+    p.f2 = fused ? 1 : 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (p.n1 != c->p_spin.n1) c->sky_valid = false;  // per-sky inputs are laid out for n1
+    c->p_spin = p;
+    c->batch = batch;
+    return TDF_OK;
+}
+
+int tdf_sincos_probe(const double *ph, int n, int mode, float *s, float *c)
+{
+    if (!ph || !s || !c || n < 1 || (mode != 0 && mode != 1)) return TDF_ERR_ARG;
+    double *dph = nullptr;
+    float *ds = nullptr, *dc = nullptr;
+    int rc = TDF_OK;
+    if (hipMalloc((void **)&dph, sizeof(double) * n) != hipSuccess ||
+        hipMalloc((void **)&ds, sizeof(float) * n) != hipSuccess ||
+        hipMalloc((void **)&dc, sizeof(float) * n) != hipSuccess) {
+        rc = TDF_ERR_NOMEM;
+    } else if (hipMemcpy(dph, ph, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+               launch_sincos_probe(dph, n, mode, ds, dc, 0) != hipSuccess ||
+               hipDeviceSynchronize() != hipSuccess ||
+               hipMemcpy(s, ds, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+               hipMemcpy(c, dc, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = TDF_ERR_HIP;
+    }
+    if (dph) (void)hipFree(dph);
+    if (ds) (void)hipFree(ds);
+    if (dc) (void)hipFree(dc);
+    return rc;
 }
 
 int tdf_profile_enable(tdf_ctx *c, int on)

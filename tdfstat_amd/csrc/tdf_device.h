@@ -135,12 +135,24 @@ __device__ __forceinline__ void dft16(cx<T> *v)
         a[q1][2] = t2;
         a[q1][3] = t3;
     }
+    // W16^k: k = 4 is a multiplication by SIGN*i, k = 2 and 6 by
+    // (+-1 + SIGN*i)/sqrt(2); the others are general complex products.
+    const T h = (T)0.70710678118654752440;
 #pragma unroll
     for (int q1 = 1; q1 < 4; q1++)
 #pragma unroll
         for (int p1 = 1; p1 < 4; p1++) {
             const int k = q1 * p1;
-            a[q1][p1] = cmul(a[q1][p1], cx<T>{C[k], (T)SIGN * S[k]});
+            const cx<T> x = a[q1][p1];
+            if (k == 4) {
+                a[q1][p1] = mul_si<SIGN>(x);
+            } else if (k == 2) {
+                a[q1][p1] = cx<T>{(x.x - (T)SIGN * x.y) * h, ((T)SIGN * x.x + x.y) * h};
+            } else if (k == 6) {
+                a[q1][p1] = cx<T>{(-x.x - (T)SIGN * x.y) * h, ((T)SIGN * x.x - x.y) * h};
+            } else {
+                a[q1][p1] = cmul(x, cx<T>{C[k], (T)SIGN * S[k]});
+            }
         }
 #pragma unroll
     for (int p1 = 0; p1 < 4; p1++) {
@@ -214,6 +226,59 @@ __device__ __forceinline__ void stockham_stage(cx<T> *tile, const cx<T> *__restr
     __syncthreads();
 }
 
+// First Stockham stage (radix 16, NS = 1, no twiddles) reading its inputs
+// through ld(column, position) instead of the LDS tile.
+template <typename T, int SIGN, int L, int NCOL, class LD>
+__device__ __forceinline__ void stockham_first_from(cx<T> *tile, LD ld)
+{
+    constexpr int NT = NCOL * L / 16;
+    constexpr int LR = L / 16;
+    constexpr int LP = ldsgeom<L>::LP;
+    const int g = threadIdx.x;
+    const int c = g / LR, j = g % LR;
+    cx<T> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = ld(c, j + r * LR);
+    dftR<16, SIGN>(v);
+#pragma unroll
+    for (int r = 0; r < 16; r++) tile[c * LP + ldsgeom<L>::pad(16 * j + r)] = v[r];
+    (void)NT;
+    __syncthreads();
+}
+
+// Last Stockham stage (radix RS, NS) leaving its outputs in registers:
+// v[u*RS + r] is output position idxD(u) + r*NS of column c(u), with
+// g = tid + u*NT, c = g / (L/RS), j = g % (L/RS), idxD = (j/NS)*NS*RS + j%NS.
+template <typename T, int SIGN, int L, int NCOL, int RS, int NS>
+__device__ __forceinline__ void stockham_last_regs(const cx<T> *tile, const cx<T> *__restrict__ twL, cx<T> *v)
+{
+    constexpr int NT = NCOL * L / 16;
+    constexpr int NB = 16 / RS;
+    constexpr int LR = L / RS;
+    constexpr int LP = ldsgeom<L>::LP;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+        const int g = tid + u * NT;
+        const int c = g / LR, j = g % LR;
+#pragma unroll
+        for (int r = 0; r < RS; r++) v[u * RS + r] = tile[c * LP + ldsgeom<L>::pad(j + r * LR)];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+        const int g = tid + u * NT;
+        const int j = g % LR;
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < RS; r++) {
+            cx<T> w = twL[r * k * (L / (NS * RS))];
+            if constexpr (SIGN > 0) w.y = -w.y;
+            v[u * RS + r] = cmul(v[u * RS + r], w);
+        }
+        dftR<RS, SIGN>(&v[u * RS]);
+    }
+}
+
 // Full in-LDS FFT of every column (natural order in, natural order out).
 // L = 16^a * {1,2,4,8}: radix-16 stages followed by one smaller stage.
 template <typename T, int SIGN, int L, int NCOL>
@@ -235,11 +300,32 @@ __device__ __forceinline__ void lds_fft(cx<T> *tile, const cx<T> *__restrict__ t
 }
 
 // ---------------------------------------------------------------------
-// Phase -> exp(-i*phase).  The phase is FP64; for T=float it is reduced
-// modulo pi/2 in FP64 (Cody-Waite, pi/2 split into two doubles) and the
-// remainder |r| <= pi/4 is evaluated with Taylor polynomials in FP32
-// (truncation < 3e-9 relative, below FP32 rounding).
+// Phase -> (sin, cos).  The phase is FP64.  For T=float it is converted to
+// revolutions and reduced to [-1/2, 1/2] in FP64 (rev - rint(rev)), and the
+// hardware sine/cosine of the fraction (v_sin_f32 / v_cos_f32, which take
+// their argument in revolutions) gives the FP32 result.
 __device__ __forceinline__ void sincos_fp64arg(double ph, float *s, float *c)
+{
+    const double INV_2PI = 0.15915494309189533577;
+    const double rev = ph * INV_2PI;
+    const float f = (float)(rev - rint(rev));
+    *s = __builtin_amdgcn_sinf(f);
+    *c = __builtin_amdgcn_cosf(f);
+}
+
+// Same with the argument already in revolutions.
+__device__ __forceinline__ void sincos_rev(double rev, float *s, float *c)
+{
+    const float f = (float)(rev - rint(rev));
+    *s = __builtin_amdgcn_sinf(f);
+    *c = __builtin_amdgcn_cosf(f);
+}
+
+// Reference FP32 evaluation: reduction modulo pi/2 in FP64 (Cody-Waite, pi/2
+// split into two doubles) and Taylor polynomials of degree 9/10 on
+// |x| <= pi/4 (truncation < 3e-9 relative).  Used by the sin/cos probe to
+// bound the hardware path.
+__device__ __forceinline__ void sincos_fp64arg_poly(double ph, float *s, float *c)
 {
     const double TWO_OVER_PI = 0.63661977236758134308;
     const double PIO2_HI = 1.5707963267948965580e+00;
@@ -249,12 +335,10 @@ __device__ __forceinline__ void sincos_fp64arg(double ph, float *s, float *c)
     r = fma(-kq, PIO2_LO, r);
     const int q = (int)((long long)kq & 3);
     const float x = (float)r, x2 = x * x;
-    // sin x = x(1 - x^2/3! + x^4/5! - x^6/7! + x^8/9!)
     float sp = fmaf(x2, 2.7557319224e-6f, -1.9841269841e-4f);
     sp = fmaf(x2, sp, 8.3333333333e-3f);
     sp = fmaf(x2, sp, -1.6666666667e-1f);
     sp = fmaf(x2 * x, sp, x);
-    // cos x = 1 - x^2/2! + x^4/4! - x^6/6! + x^8/8! - x^10/10!
     float cp = fmaf(x2, -2.7557319224e-7f, 2.4801587302e-5f);
     cp = fmaf(x2, cp, -1.3888888889e-3f);
     cp = fmaf(x2, cp, 4.1666666667e-2f);

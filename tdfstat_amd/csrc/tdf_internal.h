@@ -47,11 +47,22 @@ struct DetParams {
     double sig2, cphir, sphir;
 };
 
+// Spindown inputs (xDatma, xDatmb, tmp1) are stored in the column-major
+// order of the spindown FFT's pass 1: sample i of a detector lives at
+// (i mod n1) * n2 + i / n1 (n1*n2 = nfftf), so that a pass-1 column is
+// contiguous.
+struct SpinLayout {
+    int n1, l1, n2;
+    __host__ __device__ inline size_t at(int i) const { return (size_t)(i & (n1 - 1)) * n2 + (i >> l1); }
+};
+
 struct SkyPrepArgs {
+    SpinLayout lay;
     const float *xDat;      // [nifo][N]
     const double *DetSSB;   // [nifo][3N]
     const DetParams *det;   // [nifo]
-    double *shftf, *tmp1;   // [nifo][N] out
+    double *shftf;          // [nifo][N] out
+    double *tmp1;           // [nifo][nfftf] out, SpinLayout order
     double *partials;       // [nifo][2][nblk] out
     int N, nfft;
     double omr, oms, het0;
@@ -64,6 +75,8 @@ struct SkyPrepArgs {
 // pass 2 = n2 columns of length n1.
 struct FftPlan {
     int n, n1, n2, l1, l2;
+    int e1, e2;  // LDS tile elements of pass 1 / pass 2
+    int f2;      // spindown pass 2: fused first/last stages (needs e2 == 2*n1)
     const void *twL[2][16];  // [is_double][log2 L] -> W_L^m, m < L
     const void *tw_lo[2], *tw_hi[2];
 };
@@ -96,8 +109,8 @@ __host__ __device__ inline TmplDesc tmpl_scalars(const SkyScal &k, int iss)
 struct TrigEntry;
 
 struct SpinArgs {
-    const void *xm;       // [nifo][2][N] complex T
-    const double *tmp1;   // [nifo][N]
+    const void *xm;       // [nifo][2][nfftf] complex T, SpinLayout order
+    const double *tmp1;   // [nifo][nfftf], SpinLayout order
     SkyScal sk;
     void *Z;              // [nslots][2][n]
     void *FT;             // [nslots][n]
@@ -144,7 +157,8 @@ hipError_t launch_spline(const void *Y, void *Y2, int n, size_t ystride, size_t 
                          const double *taps, hipStream_t st);
 template <typename T>
 hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, const double *sig2, void *xm, int N,
-                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, hipStream_t st);
+                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, SpinLayout lay,
+                         size_t xmstride, hipStream_t st);
 template <typename T>
 hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st);
 template <typename T>
@@ -154,6 +168,7 @@ hipError_t launch_fstat_blocks(const void *FT, void *mu, const FstatArgs &a, int
 template <typename T>
 hipError_t launch_triggers(const void *FT, const void *mu, const TrigArgs &a, int nslots, TrigEntry *out,
                            int *counters, hipStream_t st);
+hipError_t launch_sincos_probe(const double *ph, int n, int mode, float *s, float *c, hipStream_t st);
 template <typename T>
 hipError_t launch_candidates(const void *FT, const TrigArgs &a, const TrigEntry *cand, const int *cand_cnt,
                              int cand_cap, TrigEntry *out, int *counters, hipStream_t st);

@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restri
             const double shftf = shft - shft_dot(a.ns0, a.ns1, a.ns2, ssb0);
             const double tmp1 = tmp1_of(i, shft);
             a.shftf[(size_t)d * a.N + i] = shftf;
-            a.tmp1[(size_t)d * a.N + i] = tmp1;
+            a.tmp1[(size_t)d * a.lay.n1 * a.lay.n2 + a.lay.at(i)] = tmp1;
             double aa = 0., bb = 0.;
             if (fabs(x) > 2.2250738585072014e-308) {
                 double sm, cm;
@@ -138,10 +138,10 @@ __global__ void __launch_bounds__(256) k_sky_reduce(const double *__restrict__ p
 // ---------------------------------------------------------------------
 // Generic FFT pass: cooperative load into the LDS tile, in-LDS Stockham
 // FFT of every column, cooperative store.  IO supplies load/store.
-template <typename T, int SIGN, int L, class IO>
-__global__ void __launch_bounds__(FftCfg<T>::NT) k_fft_pass(IO io, const cx<T> *__restrict__ twL)
+template <typename T, int SIGN, int L, int E, class IO>
+__global__ void __launch_bounds__(E / 16) k_fft_pass(IO io, const cx<T> *__restrict__ twL)
 {
-    constexpr int NCOL = FftCfg<T>::ELEMS / L;
+    constexpr int NCOL = E / L;
     __shared__ cx<T> tile[NCOL * ldsgeom<L>::LP];
     io.template load<SIGN, L, NCOL>(tile);
     __syncthreads();
@@ -181,7 +181,7 @@ struct IO_P1Nat {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)(2 * blockIdx.y) * n;
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
@@ -199,7 +199,7 @@ struct IO_P1Nat {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void store(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)(2 * blockIdx.y) * n;
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
@@ -223,7 +223,7 @@ struct IO_P2T {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)(2 * blockIdx.y) * n;
         const int k20 = blockIdx.x * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
@@ -236,7 +236,7 @@ struct IO_P2T {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void store(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)(2 * blockIdx.y) * n;
         const int k20 = blockIdx.x * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
@@ -251,7 +251,7 @@ struct IO_P2T {
 // Spindown pass 1: load = demodulation summed over detectors
 // (spinmod.c:490-514 semantics with an FP64 phase), store = column FFT
 // times W_nfftf^(c*k2) into Z[slot][a|b][k2*n1 + c].
-template <typename T>
+template <typename T, int NIFO>
 struct IO_SpinP1 {
     const cx<T> *xm;       // [nifo][2][N]
     const double *tmp1;    // [nifo][N]
@@ -262,21 +262,28 @@ struct IO_SpinP1 {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const TmplDesc td = tmpl_scalars(sk, sk.iss0 + slot0 + blockIdx.y);
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
+        const int nd = NIFO > 0 ? NIFO : nifo;
+        // inputs are column-major (column c is contiguous over q): lanes
+        // along q read consecutive addresses
+#pragma unroll
         for (int e = threadIdx.x; e < TC * L; e += NT) {
-            const int cl = e % TC, q = e / TC;
+            const int q = e % L, cl = e / L;
             const int i = c0 + cl + n1 * q;
+            const size_t t = (size_t)(c0 + cl) * L + q;
             cx<T> fa = {(T)0, (T)0}, fb = {(T)0, (T)0};
             if (i < N) {
                 const double ph0 = td.het1 * (double)i;
-                for (int d = 0; d < nifo; d++) {
-                    const double ph = ph0 + td.spnd * tmp1[(size_t)d * N + i];
+#pragma unroll
+                for (int d = 0; d < (NIFO > 0 ? NIFO : MAXDET); d++) {
+                    if (NIFO == 0 && d >= nd) break;
+                    const double ph = ph0 + td.spnd * tmp1[(size_t)d * n + t];
                     T s, c;
                     sincos_fp64arg(ph, &s, &c);
-                    const cx<T> ma = xm[(size_t)(2 * d) * N + i];
-                    const cx<T> mb = xm[(size_t)(2 * d + 1) * N + i];
+                    const cx<T> ma = xm[(size_t)(2 * d) * n + t];
+                    const cx<T> mb = xm[(size_t)(2 * d + 1) * n + t];
                     fa.x += ma.x * c + ma.y * s;
                     fa.y += ma.y * c - ma.x * s;
                     fb.x += mb.x * c + mb.y * s;
@@ -290,7 +297,7 @@ struct IO_SpinP1 {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void store(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)blockIdx.y * 2 * n;
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
@@ -321,7 +328,7 @@ struct IO_SpinP2 {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)blockIdx.y * 2 * n;
         const int k20 = blockIdx.x * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
@@ -334,7 +341,7 @@ struct IO_SpinP2 {
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void store(cx<T> *tile)
     {
-        constexpr int TC = NCOL / 2, NT = FftCfg<T>::NT, LP = ldsgeom<L>::LP;
+        constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)blockIdx.y * n;
         const int k20 = blockIdx.x * TC;
         const T aa = (T)sky[0], bb = (T)sky[1];
@@ -368,6 +375,58 @@ struct IO_SpinP2 {
         }
     }
 };
+
+// Pass 2 with its first Stockham stage reading Z directly from global
+// memory and its last stage computing F in registers (no LDS round trip for
+// the load and the store).  One workgroup = one column k2, arrays a and b;
+// L in {512, 1024, 2048}: stages 16, 16, L/256.
+template <typename T, int L>
+__global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T> io, const cx<T> *__restrict__ twL)
+{
+    constexpr int NCOL = 2, NT = L / 8, RS = L / 256, NB = 16 / RS, LR = L / RS;
+    __shared__ cx<T> tile[NCOL * ldsgeom<L>::LP];
+    const size_t base = (size_t)blockIdx.y * 2 * io.n;
+    const int k2 = blockIdx.x;
+    const cx<T> *zc = io.Z + base + (size_t)k2 * io.n1;
+    stockham_first_from<T, -1, L, NCOL>(tile, [&](int c, int q) { return ld_stream(zc + (size_t)c * io.n + q); });
+    stockham_stage<T, -1, L, NCOL, 16, 16>(tile, twL);
+    cx<T> v[16];
+    stockham_last_regs<T, -1, L, NCOL, RS, 256>(tile, twL, v);
+    const T aa = (T)io.sky[0], bb = (T)io.sky[1];
+    const T thrT = (T)io.thr;
+    T *ft = io.FT + (size_t)blockIdx.y * io.n + (size_t)k2 * io.n1;
+#pragma unroll
+    for (int u = 0; u < NB / 2; u++) {
+        const int j = (threadIdx.x + u * NT) % LR;
+#pragma unroll
+        for (int r = 0; r < RS; r++) {
+            const int k1 = j + r * 256;
+            const int k = k2 + io.n2 * k1;
+            bool c = false;
+            T F = 0;
+            if (k >= io.fr0 && k <= io.fr1) {
+                F = cnorm(v[u * RS + r]) / aa + cnorm(v[(u + NB / 2) * RS + r]) / bb;
+                ft[k1] = F;
+                c = io.emit_cand && k > io.fr0 && k < io.fr1 - 1 && !(F < thrT);
+            }
+            if (io.emit_cand) {
+                const unsigned long long m = __ballot(c);
+                if (m) {
+                    const int lane = threadIdx.x & 63;
+                    const int leader = __ffsll((long long)m) - 1;
+                    int bpos = 0;
+                    if (lane == leader) bpos = atomicAdd(io.cand_cnt, __popcll(m));
+                    bpos = __shfl(bpos, leader, 64);
+                    if (c) {
+                        const int pos = bpos + __popcll(m & ((1ull << lane) - 1ull));
+                        if (pos < io.cand_cap) io.cand[pos] = TrigEntry{io.slot0 + (int)blockIdx.y, k, 0.f, (float)F};
+                    }
+                }
+            }
+        }
+    }
+    (void)LR;
+}
 
 // ---------------------------------------------------------------------
 // Batched transpose of column-major FFT output: in[item][k2*n1 + k1]
@@ -472,7 +531,7 @@ __global__ void __launch_bounds__(256) k_splint(const cx<T> *__restrict__ Y, con
                                                 const double *__restrict__ shftf,
                                                 const double *__restrict__ sig2, cx<T> *__restrict__ xm,
                                                 int N, int n, int ipad, size_t ystride,
-                                                size_t y2stride)
+                                                size_t y2stride, SpinLayout lay, size_t xmstride)
 {
     const int item = blockIdx.y, d = item >> 1;
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -493,7 +552,7 @@ __global__ void __launch_bounds__(256) k_splint(const cx<T> *__restrict__ Y, con
         out = lin + cx<T>{cur.x / (T)6.0, cur.y / (T)6.0};
     }
     const T s2 = (T)sig2[d];
-    xm[(size_t)item * N + i] = cx<T>{out.x / s2, out.y / s2};
+    xm[(size_t)item * xmstride + lay.at(i)] = cx<T>{out.x / s2, out.y / s2};
 }
 
 // ---------------------------------------------------------------------
@@ -744,30 +803,62 @@ __global__ void __launch_bounds__(256) k_candidates(const T *__restrict__ FT, Tr
     }
 }
 
+// sin/cos probe: FP32-path evaluation of FP64 phases (mode 0: hardware
+// revolutions path used by the kernels, mode 1: FP64 reduction + polynomial)
+__global__ void __launch_bounds__(256) k_sincos_probe(const double *__restrict__ ph, int n, int mode,
+                                                      float *__restrict__ s, float *__restrict__ c)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float a, b;
+    if (mode == 0) sincos_fp64arg(ph[i], &a, &b);
+    else sincos_fp64arg_poly(ph[i], &a, &b);
+    s[i] = a;
+    c[i] = b;
+}
+
+hipError_t launch_sincos_probe(const double *ph, int n, int mode, float *s, float *c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sincos_probe, dim3((n + 255) / 256), dim3(256), 0, st, ph, n, mode, s, c);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------
 // launchers
 
-template <typename T, int SIGN, class IO>
-static hipError_t launch_pass(int L, IO io, dim3 grid, const cx<T> *twL, hipStream_t st)
+// E = complex elements in one workgroup's LDS tile (E/16 threads); the
+// tile holds E/L columns.  Supported E: 1024..4096 (float), 512..2048
+// (double), i.e. at most ~35 KB of LDS.
+template <typename T, int SIGN, int L, class IO>
+static hipError_t launch_pass_L(int E, IO io, dim3 grid, const cx<T> *twL, hipStream_t st)
 {
-    switch (L) {
-#define TDF_CASE(LL)                                                                                   \
-    case LL:                                                                                           \
-        if constexpr (FftCfg<T>::ELEMS / LL >= 2) {                                                    \
-            hipLaunchKernelGGL((k_fft_pass<T, SIGN, LL, IO>), grid, dim3(FftCfg<T>::NT), 0, st, io,  \
-                               twL);                                                                   \
+    constexpr int EMAX = sizeof(T) == 4 ? 4096 : 2048;
+#define TDF_ECASE(EE)                                                                                  \
+    if (E == EE) {                                                                                     \
+        if constexpr (EE <= EMAX && EE / L >= 2) {                                                     \
+            hipLaunchKernelGGL((k_fft_pass<T, SIGN, L, EE, IO>), grid, dim3(EE / 16), 0, st, io, twL); \
             return hipGetLastError();                                                                  \
         }                                                                                              \
-        break;
-        TDF_CASE(64)
-        TDF_CASE(128)
-        TDF_CASE(256)
-        TDF_CASE(512)
-        TDF_CASE(1024)
-        TDF_CASE(2048)
-#undef TDF_CASE
-    default:
-        break;
+    }
+    TDF_ECASE(512)
+    TDF_ECASE(1024)
+    TDF_ECASE(2048)
+    TDF_ECASE(4096)
+#undef TDF_ECASE
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int SIGN, class IO>
+static hipError_t launch_pass(int L, int E, IO io, dim3 grid, const cx<T> *twL, hipStream_t st)
+{
+    switch (L) {
+    case 64: return launch_pass_L<T, SIGN, 64>(E, io, grid, twL, st);
+    case 128: return launch_pass_L<T, SIGN, 128>(E, io, grid, twL, st);
+    case 256: return launch_pass_L<T, SIGN, 256>(E, io, grid, twL, st);
+    case 512: return launch_pass_L<T, SIGN, 512>(E, io, grid, twL, st);
+    case 1024: return launch_pass_L<T, SIGN, 1024>(E, io, grid, twL, st);
+    case 2048: return launch_pass_L<T, SIGN, 2048>(E, io, grid, twL, st);
+    default: break;
     }
     return hipErrorInvalidValue;
 }
@@ -795,15 +886,15 @@ hipError_t launch_fft_natural_to_T(const FftPlan &p, const void *in, void *tmp, 
     const cx<T> *twL2 = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
     IO_P1Nat<T> io1{(const cx<T> *)in, (cx<T> *)tmp, (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D],
                     (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D], p.n, p.n1, gap0, gap1};
-    const int tc1 = FftCfg<T>::ELEMS / p.n2 / 2;
-    const int tc2 = FftCfg<T>::ELEMS / p.n1 / 2;
+    const int tc1 = p.e1 / p.n2 / 2;
+    const int tc2 = p.e2 / p.n1 / 2;
     hipError_t e;
-    if (sign < 0) e = launch_pass<T, -1>(p.n2, io1, dim3(p.n1 / tc1, npairs), twL1, st);
-    else e = launch_pass<T, +1>(p.n2, io1, dim3(p.n1 / tc1, npairs), twL1, st);
+    if (sign < 0) e = launch_pass<T, -1>(p.n2, p.e1, io1, dim3(p.n1 / tc1, npairs), twL1, st);
+    else e = launch_pass<T, +1>(p.n2, p.e1, io1, dim3(p.n1 / tc1, npairs), twL1, st);
     if (e != hipSuccess) return e;
     IO_P2T<T> io2{(const cx<T> *)tmp, (cx<T> *)outT, p.n, p.n1, (T)scale};
-    if (sign < 0) e = launch_pass<T, -1>(p.n1, io2, dim3(p.n2 / tc2, npairs), twL2, st);
-    else e = launch_pass<T, +1>(p.n1, io2, dim3(p.n2 / tc2, npairs), twL2, st);
+    if (sign < 0) e = launch_pass<T, -1>(p.n1, p.e2, io2, dim3(p.n2 / tc2, npairs), twL2, st);
+    else e = launch_pass<T, +1>(p.n1, p.e2, io2, dim3(p.n2 / tc2, npairs), twL2, st);
     return e;
 }
 
@@ -828,21 +919,35 @@ hipError_t launch_spline(const void *Y, void *Y2, int n, size_t ystride, size_t 
 
 template <typename T>
 hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, const double *sig2, void *xm, int N,
-                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, hipStream_t st)
+                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, SpinLayout lay,
+                         size_t xmstride, hipStream_t st)
 {
     hipLaunchKernelGGL(k_splint<T>, dim3((N + 255) / 256, nitems), dim3(256), 0, st, (const cx<T> *)Y,
-                       (const cx<T> *)Y2, shftf, sig2, (cx<T> *)xm, N, n, ipad, ystride, y2stride);
+                       (const cx<T> *)Y2, shftf, sig2, (cx<T> *)xm, N, n, ipad, ystride, y2stride, lay,
+                       xmstride);
     return hipGetLastError();
+}
+
+template <typename T, int NIFO>
+static hipError_t launch_spin_pass1_n(const FftPlan &p, const SpinArgs &s, hipStream_t st)
+{
+    IO_SpinP1<T, NIFO> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z,
+                          (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D], (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D],
+                          s.N, p.n, p.n1, s.nifo, s.slot0};
+    const int tc1 = p.e1 / p.n2 / 2;
+    return launch_pass<T, -1>(p.n2, p.e1, io, dim3(p.n1 / tc1, s.nslots),
+                              (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2], st);
 }
 
 template <typename T>
 hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
-    IO_SpinP1<T> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z, (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D],
-                    (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D], s.N, p.n, p.n1, s.nifo, s.slot0};
-    const int tc1 = FftCfg<T>::ELEMS / p.n2 / 2;
-    return launch_pass<T, -1>(p.n2, io, dim3(p.n1 / tc1, s.nslots),
-                              (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2], st);
+    switch (s.nifo) {
+    case 1: return launch_spin_pass1_n<T, 1>(p, s, st);
+    case 2: return launch_spin_pass1_n<T, 2>(p, s, st);
+    case 3: return launch_spin_pass1_n<T, 3>(p, s, st);
+    default: return launch_spin_pass1_n<T, 0>(p, s, st);
+    }
 }
 
 template <typename T>
@@ -850,8 +955,28 @@ hipError_t launch_spin_pass2(const FftPlan &p, const SpinArgs &s, hipStream_t st
 {
     IO_SpinP2<T> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1,
                     s.emit_cand, s.slot0, s.thr, s.cand, s.cand_cnt, s.cand_cap};
-    const int tc2 = FftCfg<T>::ELEMS / p.n1 / 2;
-    return launch_pass<T, -1>(p.n1, io, dim3(p.n2 / tc2, s.nslots),
+    if (p.f2 && p.e2 == 2 * p.n1) {
+        const cx<T> *tw = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
+        const dim3 grid(p.n2, s.nslots);
+        switch (p.n1) {
+        case 512:
+            hipLaunchKernelGGL((k_spin_pass2_fused<T, 512>), grid, dim3(64), 0, st, io, tw);
+            return hipGetLastError();
+        case 1024:
+            hipLaunchKernelGGL((k_spin_pass2_fused<T, 1024>), grid, dim3(128), 0, st, io, tw);
+            return hipGetLastError();
+        case 2048:
+            if constexpr (sizeof(T) == 4) {
+                hipLaunchKernelGGL((k_spin_pass2_fused<T, 2048>), grid, dim3(256), 0, st, io, tw);
+                return hipGetLastError();
+            }
+            break;
+        default: break;
+        }
+        return hipErrorInvalidValue;
+    }
+    const int tc2 = p.e2 / p.n1 / 2;
+    return launch_pass<T, -1>(p.n1, p.e2, io, dim3(p.n2 / tc2, s.nslots),
                               (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1], st);
 }
 
@@ -895,7 +1020,8 @@ hipError_t launch_candidates(const void *FT, const TrigArgs &a, const TrigEntry 
     template hipError_t launch_spline<T>(const void *, void *, int, size_t, size_t, int, const double *,     \
                                          hipStream_t);                                                     \
     template hipError_t launch_splint<T>(const void *, const void *, const double *, const double *, void *, \
-                                         int, int, int, size_t, size_t, int, hipStream_t);                 \
+                                         int, int, int, size_t, size_t, int, SpinLayout, size_t,           \
+                                         hipStream_t);                                                     \
     template hipError_t launch_spin_pass1<T>(const FftPlan &, const SpinArgs &, hipStream_t);              \
     template hipError_t launch_spin_pass2<T>(const FftPlan &, const SpinArgs &, hipStream_t);              \
     template hipError_t launch_fstat_blocks<T>(const void *, void *, const FstatArgs &, int, hipStream_t);  \

@@ -88,7 +88,7 @@ class Context:
         self.prec = prec
         self._g = geom_from(sett)
         # spindowns per launch: enough workgroups to fill 256 CUs several times
-        self.batch = int(batch) if batch else (64 if sett.nfftf >= (1 << 19) else 128)
+        self.batch = int(batch) if batch else (128 if sett.nfftf >= (1 << 19) else 256)
         h = C.c_void_p()
         L.check(self.lib.tdf_ctx_create(C.byref(self._g), prec, self.batch, C.byref(h)), "tdf_ctx_create")
         self.h = h
@@ -185,6 +185,21 @@ class Context:
         F = np.zeros(self.sett.nfftf)
         L.check(self.lib.tdf_debug_fstat(self.h, iss, L.dptr(F)), "tdf_debug_fstat")
         return F
+
+    def autotune(self):
+        """Pick the fastest spindown plan (factorisation, tiles, batch)."""
+        L.check(self.lib.tdf_autotune(self.h), "tdf_autotune")
+        self.batch = self.plan_info()["batch"]
+        return self.plan_info()
+
+    def plan_info(self):
+        v = [C.c_int(0) for _ in range(5)]
+        L.check(self.lib.tdf_plan_info(self.h, *[C.byref(x) for x in v]), "tdf_plan_info")
+        return dict(n1=v[0].value, e1=v[1].value, e2=v[2].value, fused=v[3].value, batch=v[4].value)
+
+    def set_plan(self, n1, e1, e2, fused, batch):
+        L.check(self.lib.tdf_set_plan(self.h, n1, e1, e2, int(fused), batch), "tdf_set_plan")
+        self.batch = batch
 
     def profile(self, on=True):
         L.check(self.lib.tdf_profile_enable(self.h, int(on)), "tdf_profile_enable")

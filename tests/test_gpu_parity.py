@@ -196,3 +196,62 @@ def test_c1_two_day_network(prec, tol):
         assert len(recs) == 5
         worst = _compare_templates(ctx, orc, recs, opts, 5, tol, 20.0)
         _flush(f"c1 prec={prec} max rel F err={worst:.3e}")
+
+
+PLANS = {  # (n1, e1, e2, fused) per precision for nfftf = 2^17
+    L.TDF_PREC_FP32: [(2048, 2048, 4096, 1), (1024, 1024, 2048, 1), (512, 4096, 1024, 0), (2048, 4096, 4096, 0)],
+    L.TDF_PREC_FP64: [(1024, 512, 2048, 1), (512, 2048, 1024, 1), (256, 2048, 1024, 0)],
+}
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_c1_all_plans(prec, tol):
+    """Every spindown plan the autotuner can choose (factorisation, tile sizes,
+    fused pass 2) gives the oracle's F and triggers on the 2-day segment."""
+    rng = np.random.default_rng(11)
+    inj = synth.random_injection(rng, 12.0)
+    seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=11, injections=[inj])
+    sett = seg.sett
+    pm, mm, nn = synth.sky_to_grid(sett, inj.ra, inj.dec)
+    mm, nn = float(round(mm)), float(round(nn))
+    s0 = round(synth.spindown_to_grid(sett, inj.fdot, mm, nn))
+    opts = TS.default_opts(sett, thr=20.0, sst=s0 - 1, spndr1=s0 + 1, sstep=1)
+    orc = O.Oracle(sett, seg.dets)
+    orc.sky_stage(pm, mm, nn, opts)
+    with TS.Context(sett, seg.dets, prec=prec, batch=8) as ctx:
+        for plan in PLANS[prec]:
+            _flush(f"plan {plan} prec={prec}")
+            ctx.set_plan(*plan, 8)
+            sky, recs = ctx.job_core(pm, mm, nn, opts)
+            assert len(recs) == 3
+            _compare_templates(ctx, orc, recs, opts, 3, tol, 20.0)
+
+
+def test_autotune_returns_valid_plan():
+    seg = _segment(N=86164)
+    with TS.Context(seg.sett, seg.dets, prec=L.TDF_PREC_FP32, batch=16) as ctx:
+        plan = ctx.autotune()
+        assert plan["n1"] * (seg.sett.nfftf // plan["n1"]) == seg.sett.nfftf
+        assert 1 <= plan["batch"] <= 16
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fp32_phase_sincos_accuracy(mode):
+    """sin/cos of FP64 phases up to 1e4 rad (the spindown phases reach ~3e3
+    rad at 6 days) through the FP32 path: absolute error <= 2e-6 (mode 0,
+    hardware sin/cos of an FP64-reduced fraction of a turn) / 5e-7 (mode 1,
+    polynomial) against numpy in FP64."""
+    import ctypes as C
+    lib = L.load()
+    rng = np.random.default_rng(5)
+    ph = np.concatenate([rng.uniform(-1e4, 1e4, 200000), rng.uniform(-4, 4, 50000),
+                         np.arange(-64, 65) * (np.pi / 4)])
+    s = np.zeros(ph.size, dtype=np.float32)
+    c = np.zeros(ph.size, dtype=np.float32)
+    L.check(lib.tdf_sincos_probe(L.dptr(ph), ph.size, mode, s.ctypes.data_as(C.POINTER(C.c_float)),
+                                 c.ctypes.data_as(C.POINTER(C.c_float))), "tdf_sincos_probe")
+    es = np.abs(s - np.sin(ph)).max()
+    ec = np.abs(c - np.cos(ph)).max()
+    _flush(f"sincos mode {mode}: max abs err sin {es:.3e} cos {ec:.3e}")
+    lim = 2e-6 if mode == 0 else 5e-7
+    assert es < lim and ec < lim
