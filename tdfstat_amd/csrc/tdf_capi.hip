@@ -218,7 +218,10 @@ int make_plan(tdf_ctx *c, int n, FftPlan *p, int n1_req = 0, int e1 = 0, int e2 
     p->l2 = ilog2i(n2);
     p->e1 = e1 > 0 ? e1 : emax;
     p->e2 = e2 > 0 ? e2 : emax;
-    if (p->e1 < 2 * n2 || p->e2 < 2 * n1 || p->e1 > emax || p->e2 > emax) return TDF_ERR_UNSUPPORTED;
+    // LDS tiles: a power of two of 512..emax elements holding at least two columns
+    if (!is_pow2(p->e1) || !is_pow2(p->e2) || p->e1 < 512 || p->e2 < 512 || p->e1 < 2 * n2 ||
+        p->e2 < 2 * n1 || p->e1 > emax || p->e2 > emax)
+        return TDF_ERR_UNSUPPORTED;
     for (int d = 0; d < 2; d++)
         for (int l = 0; l < 16; l++) p->twL[d][l] = c->twL[d][l];
     int rc;

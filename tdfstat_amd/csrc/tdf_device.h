@@ -165,6 +165,53 @@ __device__ __forceinline__ void dft16(cx<T> *v)
     }
 }
 
+// R=16 with v[8..15] = 0 (only v[0..7] are read).
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft16_half(cx<T> *v)
+{
+    cx<T> w[16];
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) {
+        const cx<T> t0 = v[q1], t1 = v[q1 + 4];
+        const cx<T> s1 = mul_si<SIGN>(t1);
+        // 4-point DFT of (t0, t1, 0, 0)
+        w[q1] = t0 + t1;
+        w[q1 + 4] = t0 + s1;
+        w[q1 + 8] = t0 - t1;
+        w[q1 + 12] = t0 - s1;
+    }
+    // twiddles and outer radix-4 exactly as in dft16 (w[q1 + 4*p1] = a[q1][p1])
+    const T C[10] = {(T)1.0, (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173,
+                     (T)0.0, (T)-0.38268343236508977173, (T)-0.70710678118654752440,
+                     (T)-0.92387953251128675613, (T)-1.0, (T)-0.92387953251128675613};
+    const T S[10] = {(T)0.0, (T)0.38268343236508977173, (T)0.70710678118654752440, (T)0.92387953251128675613,
+                     (T)1.0, (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173,
+                     (T)0.0, (T)-0.38268343236508977173};
+    const T h = (T)0.70710678118654752440;
+#pragma unroll
+    for (int q1 = 1; q1 < 4; q1++)
+#pragma unroll
+        for (int p1 = 1; p1 < 4; p1++) {
+            const int k = q1 * p1;
+            const cx<T> x = w[q1 + 4 * p1];
+            cx<T> y;
+            if (k == 4) y = mul_si<SIGN>(x);
+            else if (k == 2) y = cx<T>{(x.x - (T)SIGN * x.y) * h, ((T)SIGN * x.x + x.y) * h};
+            else if (k == 6) y = cx<T>{(-x.x - (T)SIGN * x.y) * h, ((T)SIGN * x.x - x.y) * h};
+            else y = cmul(x, cx<T>{C[k], (T)SIGN * S[k]});
+            w[q1 + 4 * p1] = y;
+        }
+#pragma unroll
+    for (int p1 = 0; p1 < 4; p1++) {
+        cx<T> t0 = w[4 * p1], t1 = w[1 + 4 * p1], t2 = w[2 + 4 * p1], t3 = w[3 + 4 * p1];
+        dft4s<SIGN>(t0, t1, t2, t3);
+        v[p1] = t0;
+        v[p1 + 4] = t1;
+        v[p1 + 8] = t2;
+        v[p1 + 12] = t3;
+    }
+}
+
 template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dftR(cx<T> *v)
 {
@@ -279,13 +326,34 @@ __device__ __forceinline__ void stockham_last_regs(const cx<T> *tile, const cx<T
     }
 }
 
+// First radix-16 stage when the upper half of every column is zero: only
+// positions < L/2 of the tile are read (they need not even be written).
+template <typename T, int SIGN, int L, int NCOL>
+__device__ __forceinline__ void stockham_first_half(cx<T> *tile)
+{
+    constexpr int LR = L / 16;
+    constexpr int LP = ldsgeom<L>::LP;
+    const int g = threadIdx.x;
+    const int c = g / LR, j = g % LR;
+    cx<T> v[16];
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = tile[c * LP + ldsgeom<L>::pad(j + r * LR)];
+    __syncthreads();
+    dft16_half<SIGN>(v);
+#pragma unroll
+    for (int r = 0; r < 16; r++) tile[c * LP + ldsgeom<L>::pad(16 * j + r)] = v[r];
+    __syncthreads();
+}
+
 // Full in-LDS FFT of every column (natural order in, natural order out).
 // L = 16^a * {1,2,4,8}: radix-16 stages followed by one smaller stage.
-template <typename T, int SIGN, int L, int NCOL>
+// HALF: the upper half of every column is zero and was not stored.
+template <typename T, int SIGN, int L, int NCOL, bool HALF = false>
 __device__ __forceinline__ void lds_fft(cx<T> *tile, const cx<T> *__restrict__ twL)
 {
     static_assert(L >= 64 && L <= 4096 && (L & (L - 1)) == 0, "L");
-    stockham_stage<T, SIGN, L, NCOL, 16, 1>(tile, twL);
+    if constexpr (HALF) stockham_first_half<T, SIGN, L, NCOL>(tile);
+    else stockham_stage<T, SIGN, L, NCOL, 16, 1>(tile, twL);
     if constexpr (L == 64) {
         stockham_stage<T, SIGN, L, NCOL, 4, 16>(tile, twL);
     } else if constexpr (L == 128) {

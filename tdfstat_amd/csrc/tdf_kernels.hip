@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(E / 16) k_fft_pass(IO io, const cx<T> *__restr
     __shared__ cx<T> tile[NCOL * ldsgeom<L>::LP];
     io.template load<SIGN, L, NCOL>(tile);
     __syncthreads();
-    lds_fft<T, SIGN, L, NCOL>(tile, twL);
+    lds_fft<T, SIGN, L, NCOL, IO::HALF>(tile, twL);
     io.template store<SIGN, L, NCOL>(tile);
 }
 
@@ -174,6 +174,7 @@ __device__ __forceinline__ cx<T> twiddle_n(const cx<T> *__restrict__ lo, const c
 // Indices in [gap0, gap1) read as zero (spectrum-move gap).
 template <typename T>
 struct IO_P1Nat {
+    static constexpr bool HALF = false;
     const cx<T> *in;
     cx<T> *out;
     const cx<T> *tw_lo, *tw_hi;
@@ -216,6 +217,7 @@ struct IO_P1Nat {
 // out[k2*n1 + k1] = X[k2 + n2*k1] * scale.
 template <typename T>
 struct IO_P2T {
+    static constexpr bool HALF = false;
     const cx<T> *in;
     cx<T> *out;
     int n, n1;
@@ -251,8 +253,9 @@ struct IO_P2T {
 // Spindown pass 1: load = demodulation summed over detectors
 // (spinmod.c:490-514 semantics with an FP64 phase), store = column FFT
 // times W_nfftf^(c*k2) into Z[slot][a|b][k2*n1 + c].
-template <typename T, int NIFO>
+template <typename T, int NIFO, bool HALF_>
 struct IO_SpinP1 {
+    static constexpr bool HALF = HALF_;  // rows >= L/2 of every column are zero (N <= nfftf/2)
     const cx<T> *xm;       // [nifo][2][N]
     const double *tmp1;    // [nifo][N]
     SkyScal sk;
@@ -268,9 +271,10 @@ struct IO_SpinP1 {
         const int nd = NIFO > 0 ? NIFO : nifo;
         // inputs are column-major (column c is contiguous over q): lanes
         // along q read consecutive addresses
+        constexpr int LQ = HALF ? L / 2 : L;
 #pragma unroll
-        for (int e = threadIdx.x; e < TC * L; e += NT) {
-            const int q = e % L, cl = e / L;
+        for (int e = threadIdx.x; e < TC * LQ; e += NT) {
+            const int q = e % LQ, cl = e / LQ;
             const int i = c0 + cl + n1 * q;
             const size_t t = (size_t)(c0 + cl) * L + q;
             cx<T> fa = {(T)0, (T)0}, fb = {(T)0, (T)0};
@@ -298,14 +302,20 @@ struct IO_SpinP1 {
     __device__ __forceinline__ void store(cx<T> *tile)
     {
         constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
+        static_assert(NT % TC == 0, "tile");
         const size_t base = (size_t)blockIdx.y * 2 * n;
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
-        for (int e = threadIdx.x; e < TC * L; e += NT) {
-            const int cl = e % TC, k2 = e / TC;
-            const int c = c0 + cl;
-            const cx<T> w = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * k2));
+        // this thread stores column cl at rows k2 = k20 + u*(NT/TC): the
+        // four-step twiddle W^(c*k2) is advanced by W^(c*NT/TC) per row
+        const int cl = threadIdx.x % TC, k20 = threadIdx.x / TC;
+        const int c = c0 + cl;
+        cx<T> w = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * k20));
+        const cx<T> ws = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * (NT / TC)));
+#pragma unroll
+        for (int k2 = k20; k2 < L; k2 += NT / TC) {
             st_stream(&Z[base + (size_t)k2 * n1 + c], cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w));
             st_stream(&Z[base + n + (size_t)k2 * n1 + c], cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w));
+            w = cmul(w, ws);
         }
     }
 };
@@ -316,6 +326,7 @@ struct IO_SpinP1 {
 // to a candidate list (wavefront ballot, one atomic per wave).
 template <typename T>
 struct IO_SpinP2 {
+    static constexpr bool HALF = false;
     const cx<T> *Z;
     T *FT;
     const double *sky;  // aa, bb
@@ -928,10 +939,10 @@ hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, con
     return hipGetLastError();
 }
 
-template <typename T, int NIFO>
+template <typename T, int NIFO, bool HALF>
 static hipError_t launch_spin_pass1_n(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
-    IO_SpinP1<T, NIFO> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z,
+    IO_SpinP1<T, NIFO, HALF> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z,
                           (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D], (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D],
                           s.N, p.n, p.n1, s.nifo, s.slot0};
     const int tc1 = p.e1 / p.n2 / 2;
@@ -942,11 +953,21 @@ static hipError_t launch_spin_pass1_n(const FftPlan &p, const SpinArgs &s, hipSt
 template <typename T>
 hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
+    // N <= n1 * n2/2: the upper half of every pass-1 column is zero padding
+    const bool half = (long long)s.N <= (long long)p.n1 * (p.n2 / 2);
+    if (half) {
+        switch (s.nifo) {
+        case 1: return launch_spin_pass1_n<T, 1, true>(p, s, st);
+        case 2: return launch_spin_pass1_n<T, 2, true>(p, s, st);
+        case 3: return launch_spin_pass1_n<T, 3, true>(p, s, st);
+        default: return launch_spin_pass1_n<T, 0, true>(p, s, st);
+        }
+    }
     switch (s.nifo) {
-    case 1: return launch_spin_pass1_n<T, 1>(p, s, st);
-    case 2: return launch_spin_pass1_n<T, 2>(p, s, st);
-    case 3: return launch_spin_pass1_n<T, 3>(p, s, st);
-    default: return launch_spin_pass1_n<T, 0>(p, s, st);
+    case 1: return launch_spin_pass1_n<T, 1, false>(p, s, st);
+    case 2: return launch_spin_pass1_n<T, 2, false>(p, s, st);
+    case 3: return launch_spin_pass1_n<T, 3, false>(p, s, st);
+    default: return launch_spin_pass1_n<T, 0, false>(p, s, st);
     }
 }
 

@@ -255,3 +255,36 @@ def test_fp32_phase_sincos_accuracy(mode):
     _flush(f"sincos mode {mode}: max abs err sin {es:.3e} cos {ec:.3e}")
     lim = 2e-6 if mode == 0 else 5e-7
     assert es < lim and ec < lim
+
+
+PLANS_PAD2 = {  # nfftf = 2^18 (fftpad 2): N <= nfftf/2, so pass 1 skips the zero half of every column
+    L.TDF_PREC_FP32: [(2048, 4096, 4096, 1), (256, 4096, 512, 0), (128, 4096, 512, 0)],
+    L.TDF_PREC_FP64: [(1024, 2048, 2048, 1), (256, 2048, 512, 0)],
+}
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_fftpad2_segment(prec, tol):
+    """fftpad = 2 (the reference's default, settings.c:48): the F-statistic
+    FFT is twice the data length, and the demodulated series is zero in the
+    upper half of every pass-1 column.  F and triggers match the oracle for
+    several factorisations."""
+    rng = np.random.default_rng(12)
+    inj = synth.random_injection(rng, 12.0)
+    seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=12, injections=[inj], fftpad=2)
+    sett = seg.sett
+    assert sett.nfftf == 1 << 18 and 2 * sett.N <= sett.nfftf
+    pm, mm, nn = synth.sky_to_grid(sett, inj.ra, inj.dec)
+    mm, nn = float(round(mm)), float(round(nn))
+    s0 = round(synth.spindown_to_grid(sett, inj.fdot, mm, nn))
+    opts = TS.default_opts(sett, thr=20.0, sst=s0 - 1, spndr1=s0 + 1, sstep=1)
+    orc = O.Oracle(sett, seg.dets)
+    orc.sky_stage(pm, mm, nn, opts)
+    with TS.Context(sett, seg.dets, prec=prec, batch=8) as ctx:
+        for plan in PLANS_PAD2[prec]:
+            _flush(f"fftpad2 plan {plan} prec={prec}")
+            ctx.set_plan(*plan, 8)
+            sky, recs = ctx.job_core(pm, mm, nn, opts)
+            assert len(recs) == 3
+            worst = _compare_templates(ctx, orc, recs, opts, 3, tol, 20.0)
+            _flush(f"  max rel F err={worst:.3e}")
