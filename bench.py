@@ -94,15 +94,18 @@ def kernel_bytes(sett, S, nslots, kname, prec64):
     return None
 
 
-def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True):
+def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, plan=None):
     from tdfstat_amd import search as TS
     sett = seg.sett
     S = cfg["nspin"]
     opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
-    ctx = TS.Context(sett, seg.dets, prec=prec, batch=batch)
-    if tune:
-        plan = ctx.autotune()
-        print(f"[bench] prec={prec} autotuned plan {plan}", file=sys.stderr, flush=True)
+    ctx = TS.Context(sett, seg.dets, prec=prec, batch=max(batch, plan[4] if plan else 0))
+    if plan:
+        ctx.set_plan(*plan)
+        print(f"[bench] prec={prec} plan {ctx.plan_info()}", file=sys.stderr, flush=True)
+    elif tune:
+        p = ctx.autotune()
+        print(f"[bench] prec={prec} autotuned plan {p}", file=sys.stderr, flush=True)
     oc = opts.to_c()
     cap = (S + 1) * 64
     for k in range(warmup):
@@ -183,16 +186,56 @@ def cpu_baseline(cfg, seg, pts, budget_s=12.0):
                        f"{threads} threads, {el:.1f} s")
 
 
+def kernel_variant(kname, plan):
+    """(L, E, fused) of the spindown kernel kname under plan: the template
+    arguments that tell its compiled variants apart."""
+    n1, e1, e2, fused = plan["n1"], plan["e1"], plan["e2"], plan["fused"]
+    if kname == "spin_pass1_demod_fft":
+        return (None, e1, 0)
+    if kname == "spin_pass2_fft_fstat":
+        return (n1, e2, fused)
+    return None
+
+
+def measured_traffic(config, kname, plan, nslots):
+    """HBM bytes per launch of kname from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (bytes per template of the same
+    kernel variant, scaled to nslots templates); (None, reason) when the
+    summary does not cover this variant."""
+    path = os.path.join(ROOT, "profiles", f"spindown_kernels_{config}_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None, "no counter summary for this workload"
+    want = kernel_variant(kname, plan)
+    with open(path) as f:
+        for ln in f:
+            if not ln.startswith("variant "):
+                continue
+            kv = dict(t.split("=", 1) for t in ln.split()[1:])
+            if kv["kernel"] != kname:
+                continue
+            L_, E_, fu = int(kv["L"]), int(kv["E"]), int(kv["fused"])
+            if want[0] is None:  # pass 1: L = n2 = nfftf / n1
+                if L_ * plan["n1"] != int(kv["n"]) or E_ != want[1]:
+                    continue
+            elif (L_, E_, fu) != want:
+                continue
+            return int(float(kv["bytes_per_template"]) * nslots), os.path.relpath(path, ROOT)
+    return None, "variant not in the counter summary"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--batch", type=int, default=0, help="max spindowns per launch (0 = library default)")
     ap.add_argument("--no-tune", action="store_true", help="use the default plan instead of tdf_autotune")
+    ap.add_argument("--plan", default=None,
+                    help="FP32 plan n1,e1,e2,fused,batch instead of tdf_autotune (profiling runs)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -219,17 +262,25 @@ def main():
     print(f"[bench] config={args.config} N={sett.N} nfft={sett.nfft} nfftf={sett.nfftf} nifo={sett.nifo} "
           f"sky={len(pts)} S={S}", file=sys.stderr, flush=True)
 
+    plan = tuple(int(v) for v in args.plan.split(",")) if args.plan else None
+    if plan is not None and len(plan) != 5:
+        ap.error("--plan takes n1,e1,e2,fused,batch")
     ctx32, opts, nt, el, kt, ntrig = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
-                                             args.batch, not args.no_tune)
+                                             args.batch, not args.no_tune, plan)
     value = nt / el
     print(f"[bench] fp32: {nt} templates in {el:.3f} s -> {value:.0f} templates/s", file=sys.stderr, flush=True)
-    k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 4)
-    ctx64, _, nt64, el64, kt64, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
-                                            not args.no_tune)
-    value64 = nt64 / el64
-    print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
-          flush=True)
-    emax, emean, same = fp32_vs_fp64_error(cfg, seg, pts[0], opts, ctx32, ctx64)
+    fp64 = fperr = None
+    if not args.no_fp64:
+        k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 8)
+        ctx64, _, nt64, el64, kt64, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
+                                                not args.no_tune)
+        value64 = nt64 / el64
+        print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
+              flush=True)
+        emax, emean, same = fp32_vs_fp64_error(cfg, seg, pts[0], opts, ctx32, ctx64)
+        fp64 = {"value": round(value64, 1), "speedup_fp32_over_fp64": round(value / value64, 3),
+                "plan": ctx64.plan_info()}
+        fperr = {"max_rel_err_F": emax, "mean_rel_err_F": emean, "trigger_sets_equal": same}
 
     # roofline of the dominant spindown kernel (HIP events on the library stream)
     spin = {k: v for k, v in kt.items() if k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat", "triggers")}
@@ -239,6 +290,7 @@ def main():
     line_plan = ctx32.plan_info()
     kb = kernel_bytes(sett, S, nslots_avg, dom, False)
     ach = kb / (ms_tot / launches * 1e-3) / 1e9
+    traffic, tsrc = measured_traffic(args.config, dom, line_plan, nslots_avg)
     bt = algorithmic_bytes(sett, S, False)
     pipe = value * bt / 1e9
     line = {
@@ -259,13 +311,13 @@ def main():
                                f"noise + injections, FP64 phase",
                    "templates_timed": nt, "triggers": ntrig, "plan": line_plan},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "avg_launch_ms": round(ms_tot / launches, 5),
-                     "bytes_per_launch": int(kb)},
+                     "bytes_per_launch": int(kb), "traffic_source": tsrc},
         "pipeline_roofline": {"bytes_per_template": int(bt), "achieved": round(pipe, 1),
                               "frac": round(pipe / HBM_PEAK_GBS, 4)},
-        "fp64_path": {"value": round(value64, 1), "speedup_fp32_over_fp64": round(value / value64, 3)},
-        "fp32_vs_fp64": {"max_rel_err_F": emax, "mean_rel_err_F": emean, "trigger_sets_equal": same},
+        "fp64_path": fp64,
+        "fp32_vs_fp64": fperr,
         "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},
         "cpu_baseline": None,
     }

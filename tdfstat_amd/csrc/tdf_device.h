@@ -40,6 +40,23 @@ __device__ __forceinline__ cx<T> ld_stream(const cx<T> *p)
 {
     return cx<T>{__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y)};
 }
+// Two adjacent complex values (p[0], p[1]) as one 16-byte (float) or
+// 32-byte (double) access; p is aligned to that size.
+template <typename T>
+using vec4_t = T __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ void st_stream2(cx<T> *p, cx<T> a, cx<T> b)
+{
+    const vec4_t<T> v = {a.x, a.y, b.x, b.y};
+    __builtin_nontemporal_store(v, (vec4_t<T> *)p);
+}
+template <typename T>
+__device__ __forceinline__ void ld_stream2(const cx<T> *p, cx<T> &a, cx<T> &b)
+{
+    const vec4_t<T> v = __builtin_nontemporal_load((const vec4_t<T> *)p);
+    a = cx<T>{v.x, v.y};
+    b = cx<T>{v.z, v.w};
+}
 template <typename T>
 __device__ __forceinline__ T cnorm(cx<T> a) { return a.x * a.x + a.y * a.y; }
 
@@ -275,14 +292,18 @@ __device__ __forceinline__ void stockham_stage(cx<T> *tile, const cx<T> *__restr
 
 // First Stockham stage (radix 16, NS = 1, no twiddles) reading its inputs
 // through ld(column, position) instead of the LDS tile.
-template <typename T, int SIGN, int L, int NCOL, class LD>
+// PAIRED (NCOL = 2): adjacent lanes take the two columns of one j, so that
+// inputs stored as interleaved (column 0, column 1) pairs are read by a
+// wavefront as one contiguous run.
+template <typename T, int SIGN, int L, int NCOL, bool PAIRED = false, class LD>
 __device__ __forceinline__ void stockham_first_from(cx<T> *tile, LD ld)
 {
     constexpr int NT = NCOL * L / 16;
     constexpr int LR = L / 16;
     constexpr int LP = ldsgeom<L>::LP;
+    static_assert(!PAIRED || NCOL == 2, "paired columns");
     const int g = threadIdx.x;
-    const int c = g / LR, j = g % LR;
+    const int c = PAIRED ? (g & 1) : g / LR, j = PAIRED ? (g >> 1) : g % LR;
     cx<T> v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = ld(c, j + r * LR);

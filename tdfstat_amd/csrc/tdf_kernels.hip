@@ -252,7 +252,9 @@ struct IO_P2T {
 
 // Spindown pass 1: load = demodulation summed over detectors
 // (spinmod.c:490-514 semantics with an FP64 phase), store = column FFT
-// times W_nfftf^(c*k2) into Z[slot][a|b][k2*n1 + c].
+// times W_nfftf^(c*k2) into Z[slot][k2*n1 + c][a|b] (Fa and Fb of one bin
+// interleaved: a workgroup's TC columns then form runs of TC*2 complex
+// values, 128 bytes for TC = 8 in FP32).
 template <typename T, int NIFO, bool HALF_>
 struct IO_SpinP1 {
     static constexpr bool HALF = HALF_;  // rows >= L/2 of every column are zero (N <= nfftf/2)
@@ -313,8 +315,8 @@ struct IO_SpinP1 {
         const cx<T> ws = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * (NT / TC)));
 #pragma unroll
         for (int k2 = k20; k2 < L; k2 += NT / TC) {
-            st_stream(&Z[base + (size_t)k2 * n1 + c], cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w));
-            st_stream(&Z[base + n + (size_t)k2 * n1 + c], cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w));
+            st_stream2(&Z[base + ((size_t)k2 * n1 + c) * 2], cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w),
+                       cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w));
             w = cmul(w, ws);
         }
     }
@@ -345,8 +347,10 @@ struct IO_SpinP2 {
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int q = e % L, cl = e / L;
             const size_t off = (size_t)(k20 + cl) * n1 + q;
-            tile[cl * LP + ldsgeom<L>::pad(q)] = ld_stream(&Z[base + off]);
-            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = ld_stream(&Z[base + n + off]);
+            cx<T> a, b;
+            ld_stream2(&Z[base + off * 2], a, b);
+            tile[cl * LP + ldsgeom<L>::pad(q)] = a;
+            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = b;
         }
     }
     template <int SIGN, int L, int NCOL>
@@ -398,8 +402,8 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T> io, con
     __shared__ cx<T> tile[NCOL * ldsgeom<L>::LP];
     const size_t base = (size_t)blockIdx.y * 2 * io.n;
     const int k2 = blockIdx.x;
-    const cx<T> *zc = io.Z + base + (size_t)k2 * io.n1;
-    stockham_first_from<T, -1, L, NCOL>(tile, [&](int c, int q) { return ld_stream(zc + (size_t)c * io.n + q); });
+    const cx<T> *zc = io.Z + base + (size_t)k2 * io.n1 * 2;
+    stockham_first_from<T, -1, L, NCOL, true>(tile, [&](int c, int q) { return ld_stream(zc + (size_t)q * 2 + c); });
     stockham_stage<T, -1, L, NCOL, 16, 16>(tile, twL);
     cx<T> v[16];
     stockham_last_regs<T, -1, L, NCOL, RS, 256>(tile, twL, v);
