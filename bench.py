@@ -7,7 +7,7 @@ FP64-phase path; the plain FP64 path, the FP32-vs-FP64 F-statistic error and
 a CPU baseline (the oracle, reference PREC=FLOAT semantics, OpenMP) are
 measured in the same run.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c5|c1]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2day-2det|6day-3det|2day-2det-small]
 """
 import argparse
 import json
@@ -26,14 +26,14 @@ HBM_PEAK_GBS = 8000.0
 # Workloads of the reference configurations (realised odd x odd grids).
 CONFIGS = {
     # 2 detectors, 2 days (N=86164), nfft=2^17, 31x33 sky x 255 spindowns
-    "c2": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(31, 33), sky_step=0.15, nspin=255,
+    "2day-2det": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(31, 33), sky_step=0.15, nspin=255,
                injections=4, gaps=None, sigmas=None),
     # 3 detectors, 6 days (N=258492), nfft=2^18, fftpad=2 -> nfftf=2^19,
     # 99x101 sky x 1023 spindowns, 20% gaps
-    "c5": dict(dets=("H1", "L1", "V1"), nod=6, fftpad=2, sky=(99, 101), sky_step=0.045, nspin=1023,
+    "6day-3det": dict(dets=("H1", "L1", "V1"), nod=6, fftpad=2, sky=(99, 101), sky_step=0.045, nspin=1023,
                injections=1, gaps=(0.2, 0.2, 0.2), sigmas=None),
     # 2 detectors, 2 days, 3x5 sky x 33 spindowns, one injection
-    "c1": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(3, 5), sky_step=1.0, nspin=33,
+    "2day-2det-small": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(3, 5), sky_step=1.0, nspin=33,
                injections=1, gaps=None, sigmas=None),
 }
 
@@ -228,7 +228,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="2day-2det", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--batch", type=int, default=0, help="max spindowns per launch (0 = library default)")
