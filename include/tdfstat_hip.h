@@ -163,6 +163,69 @@ int tdf_fft_c2c(const double *in, double *out, int n, int howmany, int sign,
  * sin/cos; mode 1: FP64 reduction modulo pi/2 + FP32 polynomials). */
 int tdf_sincos_probe(const double *ph, int n, int mode, float *s, float *c);
 
+/* ---- input side: signal injection (add_signal, init.c:365-504) ---- */
+#define TDF_SIGNAL_CW 0    /* add_signal's CW model (JKS amplitudes, FP64 phase) */
+#define TDF_SIGNAL_LINE 1  /* stationary sinusoid cos(f*i + phase) in detector det only */
+typedef struct {
+    int kind;
+    int det;            /* TDF_SIGNAL_LINE: detector index                     */
+    double h0;          /* amplitude ('amp' mode of add_signal)                */
+    double snr;         /* > 0: h0 = snr / sqrt(sum_d sum_{x!=0} s_d^2/sig2_d) */
+    double fpo;         /* band start (Hz); CW frequency = fpo + f/(2 pi dt)   */
+    double f;           /* frequency within the band, rad/sample               */
+    double fdot;        /* Hz/s                                                */
+    double ra, dec, iota, psi, phase;
+    int dseg;           /* seg - reffr (init.c:389, 413-414)                   */
+} tdf_signal;
+/* Adds h0 * signal to the context's detector data where xDat != 0 (the
+ * reference skips zeros, init.c:470), starting from the data uploaded by
+ * tdf_ctx_set_detector when from_clean != 0, else from the current data.
+ * sig2 is not changed (the reference computes it before add_signal).
+ * *h0_used (may be NULL) returns the amplitude applied.  Replaces
+ * add_signal (init.c:365-504); the reference's 'snr' mode, whose h0 reads
+ * ifo[nifo] (init.c:462-464), is not reproduced. */
+int tdf_inject(tdf_ctx *ctx, const tdf_signal *s, int from_clean, double *h0_used);
+
+/* ---- trigger vetting ---- */
+/* Direct DFT sums for the vetting features of nq triggers of the current
+ * sky point: for trigger q (template iss[q] of the range tdf_sky_stage set,
+ * frequency bin bin[q]) and spindown offsets o = -2..2, the centre bin
+ * bins_out[q*5 + o+2] = bin[q] + round((sgnl0(iss) - sgnl0(iss+o)) nfftf/2pi)
+ * (same physical frequency, jobcore.c:468,571) and, per detector d and
+ * bin b = centre-4 .. centre+4,
+ *   Fa_d(b) = sum_i xDatma_d[i] e^{-i(het1 i + spnd tmp1_d[i])} e^{-2 pi i b i/nfftf}
+ * and Fb_d likewise (spinmod.c:490-514 and one bin of the DFT at
+ * jobcore.c:477), as out[(((q*5 + o+2)*nifo + d)*9 + b)*4 + {ReFa, ImFa,
+ * ReFb, ImFb}] (doubles).  F = |sum_d Fa_d|^2/aa + |sum_d Fb_d|^2/bb is the
+ * F-statistic of that bin. */
+int tdf_trigger_sums(tdf_ctx *ctx, const tdf_job_opts *o, const int *iss, const int *bin, int nq,
+                     int *bins_out, double *out);
+
+/* Vetting network 24 -> 64 -> 64 -> 1 (ReLU), logit output, trained with
+ * the mean binary cross-entropy and Adam (beta1 0.9, beta2 0.999, eps 1e-8).
+ * BF16 weights and activations, FP32 accumulation and FP32 master weights.
+ * Parameters, flat FP32: W1[64][24], b1[64], W2[64][64], b2[64], W3[64], b3. */
+#define TDF_MLP_IN 24
+#define TDF_MLP_HIDDEN 64
+#define TDF_MLP_NPARAM 5825
+typedef struct tdf_mlp tdf_mlp;
+int tdf_mlp_create(tdf_mlp **out);
+int tdf_mlp_destroy(tdf_mlp *m);
+/* sets the weights and resets the Adam state */
+int tdf_mlp_set_params(tdf_mlp *m, const float *w);
+int tdf_mlp_get_params(tdf_mlp *m, float *w);
+/* logits[n] of x[n][24] */
+int tdf_mlp_forward(tdf_mlp *m, const float *x, int n, float *logits);
+/* gradient of the mean loss over the n samples (labels t[n] in {0,1}) */
+int tdf_mlp_gradient(tdf_mlp *m, const float *x, const float *t, int n, float *grad, float *loss);
+/* epochs x (n / batch) Adam steps, minibatches in order[epoch*n + k]
+ * (each epoch's row a permutation of 0..n-1), batch a multiple of 64;
+ * loss_per_epoch[epochs] (may be NULL): mean loss over the epoch's samples */
+int tdf_mlp_train(tdf_mlp *m, const float *x, const float *t, int n, const int *order, int epochs, int batch,
+                  float lr, float *loss_per_epoch);
+/* one 16x16x16 BF16 MFMA: D[16][16] = bf16(A[16][16]) bf16(B[16][16]) (tests) */
+int tdf_mfma_probe(const float *A, const float *B, float *D);
+
 /* ---- tuning ---- */
 /* Times the spindown kernels for every supported four-step factorisation
  * n1 x n2 of nfftf, LDS tile size of both passes and spindowns per launch

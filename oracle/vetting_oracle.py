@@ -1,0 +1,149 @@
+"""CPU restatement (numpy, FP64) of the trigger-vetting path: direct DFT
+sums of a trigger, the vetting network's forward / backward pass with the
+same BF16 roundings as the GPU kernels, Adam, and ROC AUC.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker, never by the
+product path.
+
+Pins (tests/test_vetting_oracle.py): the direct DFT sums reproduce the
+F-statistic of the oracle's own FFT path (or_spindown) at every bin; the
+backward pass matches central finite differences of the forward pass in
+exact (FP64, no rounding) mode; Adam matches its closed form for a
+constant gradient; roc_auc matches brute-force pair counting.
+"""
+import math
+
+import numpy as np
+
+
+def template_scalars(sky, M, mm, nn, sstep, iss):
+    """(het1, spnd, sgnl0) of template iss: jobcore.c:430,439,465-468 with
+    ss a float (smin + iss*sstep) and the rest in double."""
+    ss = np.float32(np.float32(sky.smin) + np.float32(iss) * np.float32(sstep))
+    spnd = float(ss) * M[5] + float(np.float32(nn)) * M[9] + float(np.float32(mm)) * M[13]
+    het1 = math.fmod(float(ss) * M[4], M[0])
+    if het1 < 0:
+        het1 += M[0]
+    return het1, spnd, sky.het0 + het1
+
+
+def trigger_sums(st, sett, mm, nn, sstep, iss, bins):
+    """Per trigger (iss[q], bins[q]) and spindown offset o = -2..2: the
+    centre bin at the same physical frequency and, per detector, the DFT
+    sums Fa_d, Fb_d at centre-4 .. centre+4 of the demodulated series
+    (spinmod.c:490-514, one bin of jobcore.c:477).  st is the dict of
+    Oracle.sky_stage.  Returns (cb [nq, 5], sums [nq, 5, nifo, 9, 2])."""
+    sky, M, N, nfftf = st["sky"], sett.M, sett.N, sett.nfftf
+    nifo = st["tmp1"].shape[0]
+    i = np.arange(N, dtype=np.float64)
+    nq = len(iss)
+    cb = np.zeros((nq, 5), dtype=np.int64)
+    out = np.zeros((nq, 5, nifo, 9, 2), dtype=np.complex128)
+    for q in range(nq):
+        s0 = template_scalars(sky, M, mm, nn, sstep, iss[q])[2]
+        for o in range(5):
+            het1, spnd, sg0 = template_scalars(sky, M, mm, nn, sstep, iss[q] + o - 2)
+            c = int(bins[q]) + int(np.rint((s0 - sg0) * nfftf / (2 * math.pi)))
+            cb[q, o] = c
+            b = np.arange(c - 4, c + 5)
+            # e^{-2 pi i b i / nfftf} with the integer product reduced mod nfftf
+            k = np.mod(np.outer(b, np.arange(N, dtype=np.int64)), nfftf)
+            E = np.exp(-2j * math.pi * k / nfftf)
+            for d in range(nifo):
+                p = np.exp(-1j * (het1 * i + spnd * st["tmp1"][d]))
+                out[q, o, d, :, 0] = E @ (st["xDatma"][d] * p)
+                out[q, o, d, :, 1] = E @ (st["xDatmb"][d] * p)
+    return cb, out
+
+
+# ---------------------------------------------------------------------------
+# vetting network 24 -> 64 -> 64 -> 1
+IN, H = 24, 64
+SHAPES = [("W1", (H, IN)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (H,)), ("b3", (1,))]
+NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
+
+
+def unpack(w):
+    out, k = {}, 0
+    for name, shp in SHAPES:
+        n = int(np.prod(shp))
+        out[name] = np.asarray(w[k:k + n], dtype=np.float64).reshape(shp)
+        k += n
+    return out
+
+
+def pack(p):
+    return np.concatenate([np.asarray(p[name], dtype=np.float64).ravel() for name, _ in SHAPES])
+
+
+def bf16(a):
+    """Round to BF16 (nearest even) through FP32, returned as float64."""
+    f = np.ascontiguousarray(np.asarray(a, dtype=np.float64).astype(np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32) << 16
+    return u.view(np.float32).astype(np.float64)
+
+
+def _ident(a):
+    return np.asarray(a, dtype=np.float64)
+
+
+def loss_and_grad(w, X, t, rounded=True):
+    """Mean binary cross-entropy of logits y and its gradient (flat, same
+    layout as w).  rounded: BF16 inputs, weights and activations as the GPU
+    kernels use them (FP32 accumulation is taken as exact)."""
+    r = bf16 if rounded else _ident
+    p = unpack(w)
+    X = np.asarray(X, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    n = X.shape[0]
+    xb = r(X)
+    W1, W2, W3 = r(p["W1"]), r(p["W2"]), r(p["W3"])
+    a1 = xb @ W1.T + p["b1"]
+    h1 = r(np.maximum(a1, 0.0))
+    a2 = h1 @ W2.T + p["b2"]
+    h2 = r(np.maximum(a2, 0.0))
+    y = h2 @ W3 + p["b3"][0]
+    loss = float(np.mean(np.maximum(y, 0) - t * y + np.log1p(np.exp(-np.abs(y)))))
+    dy = (1.0 / (1.0 + np.exp(-y)) - t) / n
+    g = {"W3": dy @ h2, "b3": np.array([dy.sum()])}
+    dh2 = (a2 > 0) * (dy[:, None] * W3[None, :])
+    g["b2"] = dh2.sum(0)
+    dh2r = r(dh2)
+    g["W2"] = dh2r.T @ h1
+    dh1 = (dh2r @ W2) * (a1 > 0)
+    g["b1"] = dh1.sum(0)
+    g["W1"] = r(dh1).T @ xb
+    return loss, pack(g), y
+
+
+def forward(w, X, rounded=True):
+    return loss_and_grad(w, X, np.zeros(len(X)), rounded)[2]
+
+
+def adam(w, m, v, g, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1); step counts from 1."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** step)
+    vh = v / (1 - b2 ** step)
+    return w - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def roc_auc(scores, labels):
+    """Area under the ROC curve = P(score of a positive > score of a
+    negative) + P(tie)/2 (Mann-Whitney U), by ranks with ties averaged."""
+    s = np.asarray(scores, dtype=np.float64)
+    y = np.asarray(labels).astype(bool)
+    npos, nneg = int(y.sum()), int((~y).sum())
+    order = np.argsort(s, kind="mergesort")
+    ranks = np.empty(len(s))
+    ss = s[order]
+    k = 0
+    while k < len(s):
+        j = k
+        while j + 1 < len(s) and ss[j + 1] == ss[k]:
+            j += 1
+        ranks[order[k:j + 1]] = (k + j) / 2.0 + 1.0
+        k = j + 1
+    return float((ranks[y].sum() - npos * (npos + 1) / 2.0) / (npos * nneg))

@@ -45,6 +45,22 @@ class TdfSky(C.Structure):
     ]
 
 
+class TdfSignal(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("det", C.c_int), ("h0", C.c_double), ("snr", C.c_double),
+        ("fpo", C.c_double), ("f", C.c_double), ("fdot", C.c_double),
+        ("ra", C.c_double), ("dec", C.c_double), ("iota", C.c_double), ("psi", C.c_double),
+        ("phase", C.c_double), ("dseg", C.c_int),
+    ]
+
+
+TDF_SIGNAL_CW = 0
+TDF_SIGNAL_LINE = 1
+TDF_MLP_IN = 24
+TDF_MLP_HIDDEN = 64
+TDF_MLP_NPARAM = 5825
+
+
 class TdfRecord(C.Structure):
     _fields_ = [
         ("m", C.c_float), ("n", C.c_float), ("s", C.c_float),
@@ -56,6 +72,8 @@ class TdfRecord(C.Structure):
 # every symbol declared in include/tdfstat_hip.h: name -> (restype, argtypes)
 _P = C.c_void_p
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
 SIGNATURES = {
     "tdf_strerror": (C.c_char_p, [C.c_int]),
     "tdf_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -76,6 +94,16 @@ SIGNATURES = {
     "tdf_debug_sky_interp": (C.c_int, [_P, C.c_int, _DP, _DP]),
     "tdf_fft_c2c": (C.c_int, [_DP, _DP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "tdf_sincos_probe": (C.c_int, [_DP, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "tdf_inject": (C.c_int, [_P, C.POINTER(TdfSignal), C.c_int, _DP]),
+    "tdf_trigger_sums": (C.c_int, [_P, C.POINTER(TdfJobOpts), _IP, _IP, C.c_int, _IP, _DP]),
+    "tdf_mlp_create": (C.c_int, [C.POINTER(_P)]),
+    "tdf_mlp_destroy": (C.c_int, [_P]),
+    "tdf_mlp_set_params": (C.c_int, [_P, _FP]),
+    "tdf_mlp_get_params": (C.c_int, [_P, _FP]),
+    "tdf_mlp_forward": (C.c_int, [_P, _FP, C.c_int, _FP]),
+    "tdf_mlp_gradient": (C.c_int, [_P, _FP, _FP, C.c_int, _FP, _FP]),
+    "tdf_mlp_train": (C.c_int, [_P, _FP, _FP, C.c_int, _IP, C.c_int, C.c_int, C.c_float, _FP]),
+    "tdf_mfma_probe": (C.c_int, [_FP, _FP, _FP]),
     "tdf_autotune": (C.c_int, [_P]),
     "tdf_plan_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -124,6 +152,10 @@ def dptr(a):
 
 def fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 def fft_c2c(x, sign=-1, prec=TDF_PREC_FP64):

@@ -73,6 +73,8 @@ struct tdf_ctx {
     bool tuned = false;
     // detector data
     float *xDat = nullptr;
+    float *xDat0 = nullptr;   // data as uploaded (before tdf_inject)
+    double *sig_d = nullptr;  // [nifo][N] injected signal (tdf_inject)
     double *DetSSB = nullptr;
     DetParams *detp = nullptr;
     double *sig2_d = nullptr;
@@ -487,6 +489,7 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
     const int nifo = g->nifo, N = g->N;
     const size_t cs = c->csz;
     TRY(dmalloc(c, (void **)&c->xDat, sizeof(float) * nifo * (size_t)N));
+    TRY(dmalloc(c, (void **)&c->xDat0, sizeof(float) * nifo * (size_t)N));
     TRY(dmalloc(c, (void **)&c->DetSSB, sizeof(double) * 3 * nifo * (size_t)N));
     TRY(dmalloc(c, (void **)&c->detp, sizeof(DetParams) * nifo));
     TRY(dmalloc(c, (void **)&c->sig2_d, sizeof(double) * nifo));
@@ -546,6 +549,7 @@ int tdf_ctx_set_detector(tdf_ctx *c, int det, const float *xDat, const double *D
     if (!c || det < 0 || det >= c->g.nifo || !xDat || !DetSSB || !amod) return TDF_ERR_ARG;
     const size_t N = c->g.N;
     HIPCHK(hipMemcpy(c->xDat + det * N, xDat, sizeof(float) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->xDat0 + det * N, xDat, sizeof(float) * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->DetSSB + det * 3 * N, DetSSB, sizeof(double) * 3 * N, hipMemcpyHostToDevice));
     DetParams &dp = c->detp_h[det];
     for (int k = 0; k < 9; k++) dp.c[k] = amod[k];
@@ -1028,6 +1032,132 @@ int tdf_set_plan(tdf_ctx *c, int n1, int e1, int e2, int fused, int batch)
     if (p.n1 != c->p_spin.n1) c->sky_valid = false;  // per-sky inputs are laid out for n1
     c->p_spin = p;
     c->batch = batch;
+    return TDF_OK;
+}
+
+int tdf_inject(tdf_ctx *c, const tdf_signal *sg, int from_clean, double *h0_used)
+{
+    if (!c || !sg || (sg->kind != TDF_SIGNAL_CW && sg->kind != TDF_SIGNAL_LINE)) return TDF_ERR_ARG;
+    const tdf_geom &g = c->g;
+    const int nifo = g.nifo, N = g.N;
+    for (int d = 0; d < nifo; d++)
+        if (!c->det_set[d]) return TDF_ERR_STATE;
+    if (sg->kind == TDF_SIGNAL_LINE && (sg->det < 0 || sg->det >= nifo)) return TDF_ERR_ARG;
+    if (!c->sig_d) {
+        int rc = dmalloc(c, (void **)&c->sig_d, sizeof(double) * nifo * (size_t)N);
+        if (rc) return rc;
+    }
+    SignalArgs a{};
+    a.kind = sg->kind;
+    a.det = sg->kind == TDF_SIGNAL_LINE ? sg->det : -1;
+    a.N = N;
+    a.nifo = nifo;
+    a.omr = g.omr;
+    if (sg->kind == TDF_SIGNAL_LINE) {
+        a.f = sg->f;
+        a.phase0 = sg->phase;
+    } else {
+        // init.c:385-414 (freq in Hz = fpo + f / (2 pi dt))
+        const double freq = sg->fpo + sg->f / (2 * M_PI * g.dt);
+        double fo0 = 2 * M_PI * freq * g.dt;
+        const double fo1 = M_PI * sg->fdot * g.dt * g.dt;
+        fo0 += 2. * fo1 * N * sg->dseg;
+        const double B = 0.5 / g.dt;
+        if (fo0 / (2 * M_PI * g.dt) < sg->fpo || fo0 / (2 * M_PI * g.dt) > sg->fpo + B) return TDF_ERR_ARG;
+        a.freqo0 = fo0;
+        a.freqo1 = fo1;
+        a.phaseshift = fo0 * N * sg->dseg - fo1 * pow((double)N * sg->dseg, 2);
+        a.omsfpo = 2 * M_PI * g.dt * sg->fpo;
+        const double cosi = cos(sg->iota), psi = sg->psi, phi = sg->phase;
+        const double cosip = (1. + cosi * cosi) / 2.;
+        a.amp[0] = cos(2. * psi) * cosip * cos(phi) - sin(2. * psi) * cosi * sin(phi);
+        a.amp[1] = sin(2. * psi) * cosip * cos(phi) + cos(2. * psi) * cosi * sin(phi);
+        a.amp[2] = -cos(2. * psi) * cosip * sin(phi) - sin(2. * psi) * cosi * cos(phi);
+        a.amp[3] = -sin(2. * psi) * cosip * sin(phi) + cos(2. * psi) * cosi * cos(phi);
+        const double sa = sin(sg->ra), ca = cos(sg->ra), sd = sin(sg->dec), cd = cos(sg->dec);
+        a.ns0 = ca * cd;
+        a.ns1 = sa * cd;
+        a.ns2 = sd;
+        for (int d = 0; d < nifo; d++) {
+            const DetParams &dp = c->detp_h[d];
+            a.ms[d] = ModvirSky{cd * cd, sd * cd, sd, cd, ca * dp.cphir + sa * dp.sphir, sa * dp.cphir - ca * dp.sphir};
+        }
+    }
+    float *src = from_clean ? c->xDat0 : c->xDat;
+    a.xsrc = src;
+    a.DetSSB = c->DetSSB;
+    a.detp = c->detp;
+    a.sig = c->sig_d;
+    a.partials = c->partials;  // nifo * nblk <= 2 * nifo * prep_nblk
+    const int nblk = (N + 255) / 256;
+    double h0 = sg->h0;
+    HIPCHK(launch_signal(a, nblk, c->stream));
+    if (sg->snr > 0) {
+        std::vector<double> part((size_t)nifo * nblk);
+        HIPCHK(hipMemcpyAsync(part.data(), c->partials, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double sum = 0.;
+        for (int d = 0; d < nifo; d++) {
+            double sd = 0.;
+            for (int b = 0; b < nblk; b++) sd += part[(size_t)d * nblk + b];
+            sum += sd / c->detp_h[d].sig2;
+        }
+        if (!(sum > 0.)) return TDF_ERR_ARG;
+        h0 = sg->snr / sqrt(sum);
+    }
+    HIPCHK(launch_add_signal(src, c->sig_d, c->xDat, (size_t)nifo * N, h0, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->sky_valid = false;
+    if (h0_used) *h0_used = h0;
+    return TDF_OK;
+}
+
+int tdf_trigger_sums(tdf_ctx *c, const tdf_job_opts *o, const int *iss, const int *bin, int nq, int *bins_out,
+                     double *out)
+{
+    if (!c || !o || nq < 0 || (nq > 0 && (!iss || !bin || !bins_out || !out))) return TDF_ERR_ARG;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    if (!c->sky_valid || !c->sky.visible) return TDF_ERR_STATE;
+    if (nq == 0) return TDF_OK;
+    const tdf_geom &g = c->g;
+    const double *M = g.M;
+    const SkyScal sk{c->sky.smin, o->sstep, c->mm, c->nn, c->sky.het0, M[0], M[4], M[5], M[9], M[13], 0};
+    std::vector<int> qb((size_t)nq * 5);
+    for (int q = 0; q < nq; q++) {
+        if (iss[q] < 0 || iss[q] >= c->sky.iss_size || bin[q] < 0 || bin[q] >= g.nfftf) return TDF_ERR_ARG;
+        const double s0 = tmpl_scalars(sk, iss[q]).sgnl0;
+        for (int k = 0; k < 5; k++) {
+            const double sk_ = tmpl_scalars(sk, iss[q] + k - 2).sgnl0;
+            qb[(size_t)q * 5 + k] = bin[q] + (int)llround((s0 - sk_) * g.nfftf / (2 * M_PI));
+            bins_out[(size_t)q * 5 + k] = qb[(size_t)q * 5 + k];
+        }
+    }
+    const size_t nout = (size_t)nq * 5 * g.nifo * 36;
+    int *dq = nullptr;
+    double *dout = nullptr;
+    if (hipMalloc((void **)&dq, sizeof(int) * (size_t)nq * 6) != hipSuccess) return TDF_ERR_NOMEM;
+    if (hipMalloc((void **)&dout, sizeof(double) * nout) != hipSuccess) {
+        (void)hipFree(dq);
+        return TDF_ERR_NOMEM;
+    }
+    std::vector<int> up((size_t)nq * 6);
+    for (int q = 0; q < nq; q++) up[q] = iss[q];
+    std::copy(qb.begin(), qb.end(), up.begin() + nq);
+    SumArgs a{c->xm, c->tmp1, spin_layout(c), sk, g.N, g.nfftf, g.nifo, nq, dq, dq + nq, dout};
+    hipError_t e = hipMemcpyAsync(dq, up.data(), sizeof(int) * up.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = c->prec == TDF_PREC_FP64 ? launch_trigger_sums<double>(a, c->stream)
+                                      : launch_trigger_sums<float>(a, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dq);
+    (void)hipFree(dout);
+    if (e != hipSuccess) {
+        fprintf(stderr, "[tdfstat_hip] tdf_trigger_sums: %s\n", hipGetErrorString(e));
+        return TDF_ERR_HIP;
+    }
     return TDF_OK;
 }
 

@@ -71,6 +71,31 @@ struct SkyPrepArgs {
     double cosalfr[MAXDET], sinalfr[MAXDET];
 };
 
+// Sky-dependent constants of modvir (settings.c:254-294) for one detector.
+struct ModvirSky {
+    double c2d, c2sd, sindel, cosdel, cosalfr, sinalfr;
+};
+
+// a(t), b(t) of sample i (settings.c:274-291): zero where |x| <= DBL_MIN.
+__device__ inline void modvir_sample(const double *k, const ModvirSky &m, double omr, int i, float x, double *aa,
+                                     double *bb)
+{
+    *aa = 0.;
+    *bb = 0.;
+    if (fabs(x) > 2.2250738585072014e-308) {
+        double sm, cm;
+        sincos(omr * i, &sm, &cm);
+        const double c = m.cosalfr * cm + m.sinalfr * sm;
+        const double s = m.sinalfr * cm - m.cosalfr * sm;
+        const double c2s = 2. * c * c;
+        const double cs = c * s;
+        *aa = k[0] * (2. - m.c2d) * c2s + k[1] * (2. - m.c2d) * 2. * cs + k[2] * m.c2sd * c + k[3] * m.c2sd * s -
+              k[0] * (2. - m.c2d) + k[4] * m.c2d;
+        *bb = k[5] * m.sindel * c2s + k[6] * m.sindel * 2. * cs + k[7] * m.cosdel * c + k[8] * m.cosdel * s -
+              k[5] * m.sindel;
+    }
+}
+
 // Four-step factorisation n = n1 * n2: pass 1 = n1 columns of length n2,
 // pass 2 = n2 columns of length n1.
 struct FftPlan {
@@ -141,6 +166,41 @@ struct TrigEntry {
     int slot, ic;
     float f, F;
 };
+
+// Signal injection (add_signal, init.c:365-504) or a stationary line in
+// detector det: sig[d][i] and per-block sums of sig^2 where x != 0.
+struct SignalArgs {
+    int kind, det, N, nifo;
+    double f, phase0;                               // line: cos(f i + phase0)
+    double freqo0, freqo1, phaseshift, omsfpo;      // CW phase (init.c:385-421, 444-446)
+    double amp[4];                                  // init.c:407-410
+    double ns0, ns1, ns2, omr;
+    ModvirSky ms[MAXDET];
+    const float *xsrc;       // [nifo][N]
+    const double *DetSSB;    // [nifo][3N]
+    const DetParams *detp;   // [nifo]
+    double *sig;             // [nifo][N] out
+    double *partials;        // [nifo][nblk] out
+};
+
+// Direct DFT sums for trigger features: query q, spindown offset o-2,
+// detector d: Fa, Fb at the 9 bins qbin[q][o]-4 .. +4.
+struct SumArgs {
+    const void *xm;        // [nifo][2][nfftf] complex T, SpinLayout order
+    const double *tmp1;    // [nifo][nfftf], SpinLayout order
+    SpinLayout lay;
+    SkyScal sk;
+    int N, nfftf, nifo, nq;
+    const int *qiss;       // [nq] template index (relative to sk.smin)
+    const int *qbin;       // [nq][5] centre bin per spindown offset
+    double *out;           // [nq][5][nifo][9][4]: Re Fa, Im Fa, Re Fb, Im Fb
+};
+
+hipError_t launch_signal(const SignalArgs &a, int nblk, hipStream_t st);
+hipError_t launch_add_signal(const float *xsrc, const double *sig, float *xdst, size_t n, double h0,
+                             hipStream_t st);
+template <typename T>
+hipError_t launch_trigger_sums(const SumArgs &a, hipStream_t st);
 
 template <typename T>
 hipError_t launch_sky_prep(const SkyPrepArgs &a, void *xab, int nblk, int nifo, hipStream_t st);

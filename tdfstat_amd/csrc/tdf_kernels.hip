@@ -58,20 +58,9 @@ __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restri
             const double tmp1 = tmp1_of(i, shft);
             a.shftf[(size_t)d * a.N + i] = shftf;
             a.tmp1[(size_t)d * a.lay.n1 * a.lay.n2 + a.lay.at(i)] = tmp1;
-            double aa = 0., bb = 0.;
-            if (fabs(x) > 2.2250738585072014e-308) {
-                double sm, cm;
-                sincos(a.omr * i, &sm, &cm);
-                const double c = a.cosalfr[d] * cm + a.sinalfr[d] * sm;
-                const double s = a.sinalfr[d] * cm - a.cosalfr[d] * sm;
-                const double c2s = 2. * c * c;
-                const double cs = c * s;
-                const double *k = dp.c;
-                aa = k[0] * (2. - a.c2d) * c2s + k[1] * (2. - a.c2d) * 2. * cs +
-                     k[2] * a.c2sd * c + k[3] * a.c2sd * s - k[0] * (2. - a.c2d) + k[4] * a.c2d;
-                bb = k[5] * a.sindel * c2s + k[6] * a.sindel * 2. * cs + k[7] * a.cosdel * c +
-                     k[8] * a.cosdel * s - k[5] * a.sindel;
-            }
+            double aa, bb;
+            const ModvirSky ms{a.c2d, a.c2sd, a.sindel, a.cosdel, a.cosalfr[d], a.sinalfr[d]};
+            modvir_sample(dp.c, ms, a.omr, i, x, &aa, &bb);
             a2 = aa * aa;
             b2 = bb * bb;
             const double phase = a.het0 * i + a.oms * shft;

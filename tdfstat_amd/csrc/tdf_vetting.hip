@@ -1,0 +1,704 @@
+// Input side and trigger vetting:
+//  * signal injection into the detector data (add_signal, init.c:365-504),
+//    and a stationary one-detector line for the vetting training set;
+//  * per-trigger direct DFT sums (per detector Fa, Fb at 9 bins around the
+//    trigger and at the same physical frequency for 5 neighbouring
+//    spindowns), from which the vetting features are formed;
+//  * the vetting network 24 -> 64 -> 64 -> 1: BF16 weights and activations
+//    on MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, forward,
+//    backward and Adam, and its C ABI (tdf_mlp_*).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/tdfstat_hip.h"
+#include "tdf_internal.h"
+
+namespace tdf {
+
+// ---------------------------------------------------------------------
+// injection
+
+// One thread per sample of detector blockIdx.y: the unit-amplitude signal
+// sig[d][i] (double) and, per block, sum of sig^2 over samples with x != 0.
+__global__ void __launch_bounds__(256) k_signal(SignalArgs a)
+{
+    const int d = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double s2 = 0.;
+    if (i < a.N) {
+        const float x = a.xsrc[(size_t)d * a.N + i];
+        double s = 0.;
+        if (a.kind == TDF_SIGNAL_CW) {
+            double aa, bb;
+            modvir_sample(a.detp[d].c, a.ms[d], a.omr, i, x, &aa, &bb);
+            const double *v = a.DetSSB + (size_t)d * 3 * a.N + 3 * (size_t)i;
+            double sh = 0.;
+            {
+#pragma clang fp contract(off)
+                sh += a.ns0 * v[0];
+                sh += a.ns1 * v[1];
+                sh += a.ns2 * v[2];
+            }
+            const double di = (double)i;
+            double ph = a.freqo0 * (di + sh) + a.freqo1 * (2. * sh * di + di * di + sh * sh) + a.phaseshift;
+            ph -= a.omsfpo * di;
+            double sp, cp;
+            sincos(ph, &sp, &cp);
+            s = a.amp[0] * aa * cp + a.amp[1] * bb * cp + a.amp[2] * aa * sp + a.amp[3] * bb * sp;
+        } else if (d == a.det) {
+            s = cos(a.f * (double)i + a.phase0);
+        }
+        a.sig[(size_t)d * a.N + i] = s;
+        if (x != 0.f) s2 = s * s;
+    }
+    __shared__ double r[256];
+    r[threadIdx.x] = s2;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) r[threadIdx.x] += r[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partials[(size_t)d * gridDim.x + blockIdx.x] = r[0];
+}
+
+// xdst = xsrc + h0 * sig where xsrc != 0 (init.c:466-473)
+__global__ void __launch_bounds__(256) k_add_signal(const float *__restrict__ xsrc, const double *__restrict__ sig,
+                                                    float *__restrict__ xdst, size_t n, double h0)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const float x = xsrc[i];
+        xdst[i] = x != 0.f ? (float)((double)x + h0 * sig[i]) : x;
+    }
+}
+
+hipError_t launch_signal(const SignalArgs &a, int nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_signal, dim3(nblk, a.nifo), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_signal(const float *xsrc, const double *sig, float *xdst, size_t n, double h0, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_add_signal, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, xsrc, sig, xdst, n, h0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// per-trigger direct DFT sums
+
+__device__ __forceinline__ void turn_sincos(double rev, float *s, float *c) { sincos_rev(rev, s, c); }
+__device__ __forceinline__ void turn_sincos(double rev, double *s, double *c)
+{
+    const double f = rev - rint(rev);
+    sincos(6.283185307179586477 * f, s, c);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Block (q, o, d): Fa_d(b) = sum_i xDatma_d[i] e^{-i(het1 i + spnd tmp1_d[i])}
+// e^{-2 pi i b i / nfftf} for the 9 bins b = bin_o - 4 .. bin_o + 4 of
+// template iss_q + o - 2, likewise Fb_d (spinmod.c:490-514 followed by one
+// DFT bin, jobcore.c:472-478).
+template <typename T>
+__global__ void __launch_bounds__(256) k_trigger_sums(SumArgs a)
+{
+    const int q = blockIdx.x, o = blockIdx.y, d = blockIdx.z;
+    const TmplDesc td = tmpl_scalars(a.sk, a.qiss[q] + o - 2);
+    const long long b0 = (long long)a.qbin[q * 5 + o] - 4;
+    const cx<T> *xa = (const cx<T> *)a.xm + (size_t)(2 * d) * a.nfftf;
+    const cx<T> *xb = (const cx<T> *)a.xm + (size_t)(2 * d + 1) * a.nfftf;
+    const double *t1 = a.tmp1 + (size_t)d * a.nfftf;
+    const double inv2pi = 0.15915494309189533577, invn = 1.0 / a.nfftf;
+    T acc[9][4];
+#pragma unroll
+    for (int b = 0; b < 9; b++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[b][k] = (T)0;
+    for (int i = threadIdx.x; i < a.N; i += 256) {
+        const size_t p = a.lay.at(i);
+        long long r = (b0 * i) % a.nfftf;
+        if (r < 0) r += a.nfftf;
+        const double rev = (td.het1 * (double)i + td.spnd * t1[p]) * inv2pi + (double)r * invn;
+        T s, c, ss, cs;
+        turn_sincos(rev, &s, &c);
+        turn_sincos((double)(i & (a.nfftf - 1)) * invn, &ss, &cs);
+        cx<T> e = {c, -s};
+        const cx<T> es = {cs, -ss};
+        const cx<T> va = xa[p], vb = xb[p];
+#pragma unroll
+        for (int b = 0; b < 9; b++) {
+            const cx<T> pa = cmul(va, e), pb = cmul(vb, e);
+            acc[b][0] += pa.x;
+            acc[b][1] += pa.y;
+            acc[b][2] += pb.x;
+            acc[b][3] += pb.y;
+            e = cmul(e, es);
+        }
+    }
+    __shared__ double red[4][36];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int b = 0; b < 9; b++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double v = wave_sum((double)acc[b][k]);
+            if (lane == 0) red[wv][b * 4 + k] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < 36) {
+        const double v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        a.out[(((size_t)q * 5 + o) * a.nifo + d) * 36 + threadIdx.x] = v;
+    }
+}
+
+template <typename T>
+hipError_t launch_trigger_sums(const SumArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_trigger_sums<T>, dim3(a.nq, 5, a.nifo), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+template hipError_t launch_trigger_sums<float>(const SumArgs &, hipStream_t);
+template hipError_t launch_trigger_sums<double>(const SumArgs &, hipStream_t);
+
+// ---------------------------------------------------------------------
+// vetting network
+
+constexpr int MLP_IN = TDF_MLP_IN, MLP_INP = 32, MLP_H = TDF_MLP_HIDDEN;
+constexpr int OW1 = 0, OB1 = OW1 + MLP_H * MLP_IN, OW2 = OB1 + MLP_H, OB2 = OW2 + MLP_H * MLP_H, OW3 = OB2 + MLP_H,
+              OB3 = OW3 + MLP_H, NPARAM = OB3 + 1;
+static_assert(NPARAM == TDF_MLP_NPARAM, "parameter count");
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round to nearest even
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// BF16 copies of the weights in LDS (W1 padded to 32 inputs).
+struct alignas(16) MlpWeights {
+    unsigned short w1[MLP_H * MLP_INP];
+    unsigned short w2[MLP_H * MLP_H];
+    float w3[MLP_H];  // BF16-rounded, kept as float for the VALU dot product
+    float b1[MLP_H], b2[MLP_H], b3;
+};
+
+__device__ void load_weights(MlpWeights &W, const float *__restrict__ w)
+{
+    for (int e = threadIdx.x; e < MLP_H * MLP_INP; e += blockDim.x) {
+        const int u = e / MLP_INP, k = e % MLP_INP;
+        W.w1[e] = k < MLP_IN ? f2bf(w[OW1 + u * MLP_IN + k]) : (unsigned short)0;
+    }
+    for (int e = threadIdx.x; e < MLP_H * MLP_H; e += blockDim.x) W.w2[e] = f2bf(w[OW2 + e]);
+    for (int e = threadIdx.x; e < MLP_H; e += blockDim.x) {
+        W.w3[e] = bf2f(f2bf(w[OW3 + e]));
+        W.b1[e] = w[OB1 + e];
+        W.b2[e] = w[OB2 + e];
+    }
+    if (threadIdx.x == 0) W.b3 = w[OB3];
+}
+
+// Lane l of a wavefront: r = l % 16 (sample of the tile / MFMA column),
+// g = l / 16.  MFMA 16x16x16 operand layout: A[m][k] lane (g, r) holds
+// A[r][4g + 0..3]; B[k][n] holds B[4g + 0..3][r]; D[m][n] holds D[4g + 0..3][r].
+// Hidden activations are kept in the D layout: unit 16*mt + 4g + j, sample r,
+// which is also the B layout of the next layer's k-step mt.
+struct TileFwd {
+    bf16x4 xb[2];   // inputs, B layout (features 16kt + 4g + j)
+    float a1[4][4]; // layer-1 pre-activations (unit 16mt + 4g + j)
+    bf16x4 h1[4];   // relu(a1) in BF16
+    float a2[4][4];
+    bf16x4 h2[4];
+    float y;        // logit of sample r
+};
+
+__device__ __forceinline__ void tile_forward(const MlpWeights &W, const float *__restrict__ X, int s, bool valid,
+                                             TileFwd &t)
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int kt = 0; kt < 2; kt++) {
+        const int f0 = 16 * kt + 4 * g;
+        float4 v = {0.f, 0.f, 0.f, 0.f};
+        if (valid && f0 < MLP_IN) v = *(const float4 *)(X + (size_t)s * MLP_IN + f0);
+        t.xb[kt] = bf16x4{(short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w)};
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 2; kt++)
+            acc = mfma16(*(const bf16x4 *)&W.w1[(16 * mt + r) * MLP_INP + 16 * kt + 4 * g], t.xb[kt], acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            t.a1[mt][j] = acc[j] + W.b1[16 * mt + 4 * g + j];
+            t.h1[mt][j] = (short)f2bf(fmaxf(t.a1[mt][j], 0.f));
+        }
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+            acc = mfma16(*(const bf16x4 *)&W.w2[(16 * mt + r) * MLP_H + 16 * kt + 4 * g], t.h1[kt], acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int u = 16 * mt + 4 * g + j;
+            t.a2[mt][j] = acc[j] + W.b2[u];
+            const unsigned short h = f2bf(fmaxf(t.a2[mt][j], 0.f));
+            t.h2[mt][j] = (short)h;
+            part += W.w3[u] * bf2f(h);
+        }
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    t.y = part + W.b3;
+}
+
+// logits of samples [64*blockIdx.x, ...): 4 wavefronts x 16 samples
+__global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ w, const float *__restrict__ X, int n,
+                                                     float *__restrict__ out)
+{
+    __shared__ MlpWeights W;
+    load_weights(W, w);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, r = lane & 15;
+    const int s = blockIdx.x * 64 + (threadIdx.x >> 6) * 16 + r;
+    TileFwd t;
+    tile_forward(W, X, s, s < n, t);
+    if (lane < 16 && s < n) out[s] = t.y;
+}
+
+struct MlpTrainArgs {
+    const float *X;     // [n][24]
+    const float *t;     // [n] labels 0/1
+    const int *order;   // [epochs][n] sample order (train mode)
+    int n, batch, epochs, mode;  // mode 0: gradient of the mean loss over all n samples; 1: Adam training
+    int step0;          // Adam steps taken before this call
+    float lr;
+    float *w, *m, *v;   // [NPARAM] FP32 master weights, Adam moments
+    float *grad;        // [NPARAM] (mode 0)
+    float *loss;        // [epochs] mean loss (mode 1) or [1] (mode 0)
+};
+
+constexpr float ADAM_B1 = 0.9f, ADAM_B2 = 0.999f, ADAM_EPS = 1e-8f;
+
+// One workgroup (4 wavefronts) runs every step: per step each wavefront
+// takes batch/64 tiles of 16 samples; the per-wavefront gradients are summed
+// in a fixed order in LDS; then one Adam update of all parameters.
+__global__ void __launch_bounds__(256) k_mlp_train(MlpTrainArgs a)
+{
+    __shared__ MlpWeights W;
+    __shared__ float G[NPARAM];
+    // per wavefront [sample][unit] staging; Hs holds H1, then dH1
+    __shared__ unsigned short Xs[4][16][MLP_INP], Hs[4][16][MLP_H], D2s[4][16][MLP_H];
+    __shared__ float lred[4];
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
+    load_weights(W, a.w);
+    __syncthreads();
+    const int steps = a.mode == 0 ? 1 : a.n / a.batch;
+    const int ntile = a.mode == 0 ? (a.n + 63) / 64 : a.batch / 64;
+    const float invB = a.mode == 0 ? 1.f / a.n : 1.f / a.batch;
+    const int epochs = a.mode == 0 ? 1 : a.epochs;
+    int t_adam = a.step0;
+    for (int ep = 0; ep < epochs; ep++) {
+        float lsum = 0.f;
+        for (int st = 0; st < steps; st++) {
+            f32x4 gw2[4][4], gw1[4][2];
+            float gw3[4][4], gb2[4][4], gb1[4][4], gb3 = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    gw2[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    gw3[i][j] = gb2[i][j] = gb1[i][j] = 0.f;
+                }
+                gw1[i][0] = gw1[i][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+            for (int tl = 0; tl < ntile; tl++) {
+                const int pos = (tl * 4 + wv) * 16 + r;  // position within the step's samples
+                bool valid;
+                int s;
+                if (a.mode == 0) {
+                    valid = pos < a.n;
+                    s = pos;
+                } else {
+                    valid = true;
+                    s = a.order[(size_t)ep * a.n + (size_t)st * a.batch + pos];
+                }
+                TileFwd t;
+                tile_forward(W, a.X, s, valid, t);
+                const float tgt = valid ? a.t[s] : 0.f;
+                const float sg = 1.f / (1.f + expf(-t.y));
+                const float dy = valid ? (sg - tgt) * invB : 0.f;
+                if (valid && g == 0)
+                    lsum += fmaxf(t.y, 0.f) - tgt * t.y + log1pf(expf(-fabsf(t.y)));
+                if (g == 0) gb3 += dy;
+                bf16x4 d2[4];
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int u = 16 * mt + 4 * g + j;
+                        gw3[mt][j] += dy * bf2f((unsigned short)t.h2[mt][j]);
+                        const float dh = t.a2[mt][j] > 0.f ? W.w3[u] * dy : 0.f;
+                        gb2[mt][j] += dh;
+                        d2[mt][j] = (short)f2bf(dh);
+                    }
+                // dH1 = W2^T dH2 (A[m][k] = W2[k][m])
+                bf16x4 d1[4];
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++) {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kt = 0; kt < 4; kt++) {
+                        bf16x4 A;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) A[j] = (short)W.w2[(16 * kt + 4 * g + j) * MLP_H + 16 * mt + r];
+                        acc = mfma16(A, d2[kt], acc);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float dh = t.a1[mt][j] > 0.f ? acc[j] : 0.f;
+                        gb1[mt][j] += dh;
+                        d1[mt][j] = (short)f2bf(dh);
+                    }
+                }
+                // stage [sample][unit] tiles for the weight gradients (K = samples)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+#pragma unroll
+                    for (int kt = 0; kt < 2; kt++) Xs[wv][r][16 * kt + 4 * g + j] = (unsigned short)t.xb[kt][j];
+#pragma unroll
+                    for (int mt = 0; mt < 4; mt++) {
+                        Hs[wv][r][16 * mt + 4 * g + j] = (unsigned short)t.h1[mt][j];
+                        D2s[wv][r][16 * mt + 4 * g + j] = (unsigned short)d2[mt][j];
+                    }
+                }
+                __syncthreads();
+                // dW2 += dH2 H1^T
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++) {
+                    bf16x4 A2;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) A2[j] = (short)D2s[wv][4 * g + j][16 * mt + r];
+#pragma unroll
+                    for (int nt = 0; nt < 4; nt++) {
+                        bf16x4 B;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) B[j] = (short)Hs[wv][4 * g + j][16 * nt + r];
+                        gw2[mt][nt] = mfma16(A2, B, gw2[mt][nt]);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int mt = 0; mt < 4; mt++) Hs[wv][r][16 * mt + 4 * g + j] = (unsigned short)d1[mt][j];
+                __syncthreads();
+                // dW1 += dH1 X^T
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++) {
+                    bf16x4 A1;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) A1[j] = (short)Hs[wv][4 * g + j][16 * mt + r];
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++) {
+                        bf16x4 B;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) B[j] = (short)Xs[wv][4 * g + j][16 * nt + r];
+                        gw1[mt][nt] = mfma16(A1, B, gw1[mt][nt]);
+                    }
+                }
+                __syncthreads();
+            }
+            // sums over the 16 samples of a lane group (lanes with equal g)
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int m = 1; m < 16; m <<= 1) {
+                        gw3[mt][j] += __shfl_xor(gw3[mt][j], m, 64);
+                        gb2[mt][j] += __shfl_xor(gb2[mt][j], m, 64);
+                        gb1[mt][j] += __shfl_xor(gb1[mt][j], m, 64);
+                    }
+            gb3 = wave_sum(gb3);
+            // wavefront 0 writes, 1..3 add, in order
+            for (int w = 0; w < 4; w++) {
+                if (wv == w) {
+#pragma unroll
+                    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const int u = 16 * mt + 4 * g + j;
+#pragma unroll
+                            for (int nt = 0; nt < 4; nt++) {
+                                float &dst = G[OW2 + u * MLP_H + 16 * nt + r];
+                                dst = (w == 0 ? 0.f : dst) + gw2[mt][nt][j];
+                            }
+#pragma unroll
+                            for (int nt = 0; nt < 2; nt++) {
+                                const int k = 16 * nt + r;
+                                if (k < MLP_IN) {
+                                    float &dst = G[OW1 + u * MLP_IN + k];
+                                    dst = (w == 0 ? 0.f : dst) + gw1[mt][nt][j];
+                                }
+                            }
+                            if (r == 0) {
+                                G[OW3 + u] = (w == 0 ? 0.f : G[OW3 + u]) + gw3[mt][j];
+                                G[OB2 + u] = (w == 0 ? 0.f : G[OB2 + u]) + gb2[mt][j];
+                                G[OB1 + u] = (w == 0 ? 0.f : G[OB1 + u]) + gb1[mt][j];
+                            }
+                        }
+                    if (lane == 0) G[OB3] = (w == 0 ? 0.f : G[OB3]) + gb3;
+                }
+                __syncthreads();
+            }
+            if (a.mode == 0) {
+                for (int p = threadIdx.x; p < NPARAM; p += 256) a.grad[p] = G[p];
+            } else {
+                t_adam++;
+                const float bc1 = (float)(1.0 - pow((double)ADAM_B1, (double)t_adam));
+                const float bc2 = (float)(1.0 - pow((double)ADAM_B2, (double)t_adam));
+                for (int p = threadIdx.x; p < NPARAM; p += 256) {
+                    const float gr = G[p];
+                    const float m = ADAM_B1 * a.m[p] + (1.f - ADAM_B1) * gr;
+                    const float v = ADAM_B2 * a.v[p] + (1.f - ADAM_B2) * gr * gr;
+                    a.m[p] = m;
+                    a.v[p] = v;
+                    const float wn = a.w[p] - a.lr * (m / bc1) / (sqrtf(v / bc2) + ADAM_EPS);
+                    a.w[p] = wn;
+                    if (p < OB1) {
+                        W.w1[(p / MLP_IN) * MLP_INP + p % MLP_IN] = f2bf(wn);
+                    } else if (p < OW2) {
+                        W.b1[p - OB1] = wn;
+                    } else if (p < OB2) {
+                        W.w2[p - OW2] = f2bf(wn);
+                    } else if (p < OW3) {
+                        W.b2[p - OB2] = wn;
+                    } else if (p < OB3) {
+                        W.w3[p - OW3] = bf2f(f2bf(wn));
+                    } else {
+                        W.b3 = wn;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // mean loss of the epoch
+        lsum = wave_sum(lsum);
+        if (lane == 0) lred[wv] = lsum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float nseen = a.mode == 0 ? (float)a.n : (float)(steps * a.batch);
+            a.loss[ep] = (lred[0] + lred[1] + lred[2] + lred[3]) / nseen;
+        }
+        __syncthreads();
+    }
+}
+
+// MFMA lane-layout probe: D = A(16x16) B(16x16) with one 16x16x16 BF16 MFMA.
+__global__ void __launch_bounds__(64) k_mfma_probe(const float *__restrict__ A, const float *__restrict__ B,
+                                                   float *__restrict__ D)
+{
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    bf16x4 a, b;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        a[j] = (short)f2bf(A[r * 16 + 4 * g + j]);
+        b[j] = (short)f2bf(B[(4 * g + j) * 16 + r]);
+    }
+    const f32x4 d = mfma16(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+    for (int j = 0; j < 4; j++) D[(4 * g + j) * 16 + r] = d[j];
+}
+
+}  // namespace tdf
+
+using namespace tdf;
+
+struct tdf_mlp {
+    float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *loss = nullptr;
+    int steps = 0;
+    hipStream_t stream = nullptr;
+};
+
+#define MLPCHK(x)                                                                            \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) {                                                              \
+            fprintf(stderr, "[tdfstat_hip] %s failed: %s\n", #x, hipGetErrorString(e_));     \
+            return TDF_ERR_HIP;                                                              \
+        }                                                                                    \
+    } while (0)
+
+namespace {
+
+// device copy of a host array, freed by the destructor
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int put(const void *src, size_t bytes, hipStream_t st)
+    {
+        if (hipMalloc(&p, bytes ? bytes : 4) != hipSuccess) {
+            p = nullptr;
+            return TDF_ERR_NOMEM;
+        }
+        if (src && bytes && hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+            return TDF_ERR_HIP;
+        return TDF_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int tdf_mlp_create(tdf_mlp **out)
+{
+    if (!out) return TDF_ERR_ARG;
+    tdf_mlp *m = new tdf_mlp();
+    const size_t b = sizeof(float) * NPARAM;
+    if (hipMalloc((void **)&m->w, b) != hipSuccess || hipMalloc((void **)&m->m, b) != hipSuccess ||
+        hipMalloc((void **)&m->v, b) != hipSuccess || hipMalloc((void **)&m->grad, b) != hipSuccess ||
+        hipMalloc((void **)&m->loss, sizeof(float) * 4096) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
+        tdf_mlp_destroy(m);
+        return TDF_ERR_NOMEM;
+    }
+    if (hipMemset(m->w, 0, b) != hipSuccess || hipMemset(m->m, 0, b) != hipSuccess ||
+        hipMemset(m->v, 0, b) != hipSuccess) {
+        tdf_mlp_destroy(m);
+        return TDF_ERR_HIP;
+    }
+    *out = m;
+    return TDF_OK;
+}
+
+int tdf_mlp_destroy(tdf_mlp *m)
+{
+    if (!m) return TDF_ERR_ARG;
+    for (float *p : {m->w, m->m, m->v, m->grad, m->loss})
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return TDF_OK;
+}
+
+int tdf_mlp_set_params(tdf_mlp *m, const float *w)
+{
+    if (!m || !w) return TDF_ERR_ARG;
+    const size_t b = sizeof(float) * NPARAM;
+    MLPCHK(hipMemcpy(m->w, w, b, hipMemcpyHostToDevice));
+    MLPCHK(hipMemset(m->m, 0, b));
+    MLPCHK(hipMemset(m->v, 0, b));
+    m->steps = 0;
+    return TDF_OK;
+}
+
+int tdf_mlp_get_params(tdf_mlp *m, float *w)
+{
+    if (!m || !w) return TDF_ERR_ARG;
+    MLPCHK(hipStreamSynchronize(m->stream));
+    MLPCHK(hipMemcpy(w, m->w, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    return TDF_OK;
+}
+
+int tdf_mlp_forward(tdf_mlp *m, const float *x, int n, float *logits)
+{
+    if (!m || n < 0 || (n > 0 && (!x || !logits))) return TDF_ERR_ARG;
+    if (n == 0) return TDF_OK;
+    DevBuf dx, dy;
+    int rc;
+    if ((rc = dx.put(x, sizeof(float) * MLP_IN * (size_t)n, m->stream))) return rc;
+    if ((rc = dy.put(nullptr, sizeof(float) * (size_t)n, m->stream))) return rc;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((n + 63) / 64), dim3(256), 0, m->stream, m->w, (const float *)dx.p, n,
+                       (float *)dy.p);
+    MLPCHK(hipGetLastError());
+    MLPCHK(hipMemcpyAsync(logits, dy.p, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+    MLPCHK(hipStreamSynchronize(m->stream));
+    return TDF_OK;
+}
+
+int tdf_mlp_gradient(tdf_mlp *m, const float *x, const float *t, int n, float *grad, float *loss)
+{
+    if (!m || !x || !t || !grad || n < 1) return TDF_ERR_ARG;
+    DevBuf dx, dt;
+    int rc;
+    if ((rc = dx.put(x, sizeof(float) * MLP_IN * (size_t)n, m->stream))) return rc;
+    if ((rc = dt.put(t, sizeof(float) * (size_t)n, m->stream))) return rc;
+    MlpTrainArgs a{(const float *)dx.p, (const float *)dt.p, nullptr, n, 64, 1, 0, m->steps, 0.f,
+                   m->w, m->m, m->v, m->grad, m->loss};
+    hipLaunchKernelGGL(k_mlp_train, dim3(1), dim3(256), 0, m->stream, a);
+    MLPCHK(hipGetLastError());
+    MLPCHK(hipMemcpyAsync(grad, m->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, m->stream));
+    float l = 0.f;
+    MLPCHK(hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MLPCHK(hipStreamSynchronize(m->stream));
+    if (loss) *loss = l;
+    return TDF_OK;
+}
+
+int tdf_mlp_train(tdf_mlp *m, const float *x, const float *t, int n, const int *order, int epochs, int batch,
+                  float lr, float *loss_per_epoch)
+{
+    if (!m || !x || !t || !order || epochs < 1 || epochs > 4096 || batch < 64 || batch % 64 || n < batch)
+        return TDF_ERR_ARG;
+    for (size_t k = 0; k < (size_t)epochs * n; k++)
+        if (order[k] < 0 || order[k] >= n) return TDF_ERR_ARG;
+    DevBuf dx, dt, dord;
+    int rc;
+    if ((rc = dx.put(x, sizeof(float) * MLP_IN * (size_t)n, m->stream))) return rc;
+    if ((rc = dt.put(t, sizeof(float) * (size_t)n, m->stream))) return rc;
+    if ((rc = dord.put(order, sizeof(int) * (size_t)epochs * n, m->stream))) return rc;
+    MlpTrainArgs a{(const float *)dx.p, (const float *)dt.p, (const int *)dord.p, n, batch, epochs, 1, m->steps, lr,
+                   m->w, m->m, m->v, m->grad, m->loss};
+    hipLaunchKernelGGL(k_mlp_train, dim3(1), dim3(256), 0, m->stream, a);
+    MLPCHK(hipGetLastError());
+    if (loss_per_epoch)
+        MLPCHK(hipMemcpyAsync(loss_per_epoch, m->loss, sizeof(float) * epochs, hipMemcpyDeviceToHost, m->stream));
+    MLPCHK(hipStreamSynchronize(m->stream));
+    m->steps += epochs * (n / batch);
+    return TDF_OK;
+}
+
+int tdf_mfma_probe(const float *A, const float *B, float *D)
+{
+    if (!A || !B || !D) return TDF_ERR_ARG;
+    DevBuf da, db, dd;
+    int rc;
+    if ((rc = da.put(A, sizeof(float) * 256, nullptr))) return rc;
+    if ((rc = db.put(B, sizeof(float) * 256, nullptr))) return rc;
+    if ((rc = dd.put(nullptr, sizeof(float) * 256, nullptr))) return rc;
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, nullptr, (const float *)da.p, (const float *)db.p,
+                       (float *)dd.p);
+    MLPCHK(hipGetLastError());
+    MLPCHK(hipMemcpy(D, dd.p, sizeof(float) * 256, hipMemcpyDeviceToHost));
+    return TDF_OK;
+}
+
+}  // extern "C"

@@ -166,6 +166,33 @@ class Context:
                                       C.byref(npairs)), "tdf_job_core")
         return sky, nrec.value, npairs.value
 
+    # -- input side --
+    def inject(self, kind, *, h0=0.0, snr=0.0, f=0.0, fdot=0.0, ra=0.0, dec=0.0, iota=0.0, psi=0.0,
+               phase=0.0, det=0, dseg=0, from_clean=True):
+        """add_signal (init.c:365-504) on the device copy of the data:
+        kind L.TDF_SIGNAL_CW (JKS signal) or L.TDF_SIGNAL_LINE (cos(f i +
+        phase) in detector det).  h0 ('amp' mode) or snr > 0 (h0 from the
+        network SNR over sig2).  Returns the h0 applied."""
+        sg = L.TdfSignal(kind=kind, det=det, h0=h0, snr=snr, fpo=self.sett.fpo, f=f, fdot=fdot, ra=ra, dec=dec,
+                         iota=iota, psi=psi, phase=phase, dseg=dseg)
+        h = C.c_double(0)
+        L.check(self.lib.tdf_inject(self.h, C.byref(sg), int(from_clean), C.byref(h)), "tdf_inject")
+        return h.value
+
+    def trigger_sums(self, opts, iss, bins):
+        """Direct DFT sums of triggers (iss[q], bins[q]) of the current sky
+        point: (centre bins [nq, 5], sums [nq, 5, nifo, 9, 2] complex
+        (Fa, Fb))."""
+        iss = np.ascontiguousarray(iss, dtype=np.int32)
+        bins = np.ascontiguousarray(bins, dtype=np.int32)
+        nq = iss.size
+        o = opts.to_c()
+        cb = np.zeros((nq, 5), dtype=np.int32)
+        out = np.zeros((nq, 5, self.sett.nifo, 9, 4))
+        L.check(self.lib.tdf_trigger_sums(self.h, C.byref(o), L.iptr(iss), L.iptr(bins), nq, L.iptr(cb),
+                                          L.dptr(out)), "tdf_trigger_sums")
+        return cb, out.view(np.complex128)
+
     # -- inspection --
     def sky_arrays(self, det):
         N = self.sett.N

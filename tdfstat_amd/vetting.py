@@ -1,0 +1,285 @@
+"""Trigger vetting: features, the BF16 network on the GPU, the training-set
+generator and the evaluation.
+
+The F-statistic threshold list of job_core is produced first and is never
+changed here; the network only attaches a score to each trigger (astrophysical
+signal vs single-detector instrumental line) so triggers can be ranked.
+
+Features (24 per trigger, two detectors), from the direct DFT sums of
+``Context.trigger_sums`` (per detector Fa, Fb at the trigger bin, at the 8
+neighbouring bins and at the same physical frequency for spindowns ss-2..ss+2):
+
+  0-3   log1p |Fa_d|^2/aa, log1p |Fb_d|^2/bb for d = 0, 1 (trigger bin)
+  4-12  log1p F at bins ic-4 .. ic+4 (network F, the pipeline's statistic)
+  13-17 log1p F at spindowns ss-2 .. ss+2, same physical frequency
+  18    least-squares slope of features 13-17 over the spindown offset
+  19    detector balance F_0 / (F_0 + F_1), F_d = |Fa_d|^2/aa + |Fb_d|^2/bb
+  20    coherence F / (F_0 + F_1)
+  21-22 cos of the phase between the detectors' Fa, and between their Fb
+  23    log of F over the mean F of the 8 neighbouring bins
+
+  python -m tdfstat_amd.vetting [--injections 20000] [--lines 20000] [--epochs 30]
+prints a JSON report (ROC AUC of the network and of F alone on held-out events).
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import search as TS
+from . import synth
+
+NFEAT = L.TDF_MLP_IN
+FEATURE_NAMES = (["log1p|Fa0|2/aa", "log1p|Fb0|2/bb", "log1p|Fa1|2/aa", "log1p|Fb1|2/bb"]
+                 + [f"log1pF[ic{k:+d}]" for k in range(-4, 5)]
+                 + [f"log1pF[ss{k:+d}]" for k in range(-2, 3)]
+                 + ["spindown slope", "detector balance", "coherence", "cos dphi_a", "cos dphi_b",
+                    "peak contrast"])
+assert len(FEATURE_NAMES) == NFEAT
+
+
+def features(sums, aa, bb):
+    """sums [nq, 5, 2, 9, 2] complex (offset, detector, bin, a|b) -> [nq, 24] float32."""
+    sums = np.asarray(sums)
+    if sums.ndim != 5 or sums.shape[1:] != (5, 2, 9, 2):
+        raise ValueError("features are defined for two detectors: sums must be [nq, 5, 2, 9, 2]")
+    Fa = sums[..., 0].sum(axis=2)  # [nq, 5, 9] network
+    Fb = sums[..., 1].sum(axis=2)
+    F = np.abs(Fa) ** 2 / aa + np.abs(Fb) ** 2 / bb
+    c = sums[:, 2, :, 4, :]        # [nq, det, a|b] at the trigger
+    pa = np.abs(c[:, :, 0]) ** 2 / aa
+    pb = np.abs(c[:, :, 1]) ** 2 / bb
+    Fd = pa + pb
+    out = np.zeros((sums.shape[0], NFEAT))
+    out[:, 0], out[:, 1], out[:, 2], out[:, 3] = (np.log1p(pa[:, 0]), np.log1p(pb[:, 0]),
+                                                  np.log1p(pa[:, 1]), np.log1p(pb[:, 1]))
+    out[:, 4:13] = np.log1p(F[:, 2, :])
+    out[:, 13:18] = np.log1p(F[:, :, 4])
+    off = np.arange(-2, 3)
+    out[:, 18] = out[:, 13:18] @ off / float(off @ off)
+    tiny = 1e-30
+    out[:, 19] = Fd[:, 0] / (Fd.sum(1) + tiny)
+    out[:, 20] = F[:, 2, 4] / (Fd.sum(1) + tiny)
+    za = c[:, 0, 0] * np.conj(c[:, 1, 0])
+    zb = c[:, 0, 1] * np.conj(c[:, 1, 1])
+    out[:, 21] = np.real(za) / (np.abs(za) + tiny)
+    out[:, 22] = np.real(zb) / (np.abs(zb) + tiny)
+    nb = (F[:, 2, :].sum(1) - F[:, 2, 4]) / 8.0
+    out[:, 23] = np.log((F[:, 2, 4] + tiny) / (nb + tiny))
+    return out.astype(np.float32)
+
+
+class Mlp:
+    """The 24 -> 64 -> 64 -> 1 vetting network on the GPU (tdf_mlp_*)."""
+
+    def __init__(self, params=None, seed=0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        L.check(self.lib.tdf_mlp_create(C.byref(h)), "tdf_mlp_create")
+        self.h = h
+        self.set_params(init_params(seed) if params is None else params)
+
+    def close(self):
+        if self.h:
+            self.lib.tdf_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == L.TDF_MLP_NPARAM
+        L.check(self.lib.tdf_mlp_set_params(self.h, L.fptr(w)), "tdf_mlp_set_params")
+
+    def params(self):
+        w = np.zeros(L.TDF_MLP_NPARAM, dtype=np.float32)
+        L.check(self.lib.tdf_mlp_get_params(self.h, L.fptr(w)), "tdf_mlp_get_params")
+        return w
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        L.check(self.lib.tdf_mlp_forward(self.h, L.fptr(X), X.shape[0], L.fptr(y)), "tdf_mlp_forward")
+        return y
+
+    def gradient(self, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(L.TDF_MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float(0)
+        L.check(self.lib.tdf_mlp_gradient(self.h, L.fptr(X), L.fptr(t), X.shape[0], L.fptr(g), C.byref(loss)),
+                "tdf_mlp_gradient")
+        return g, loss.value
+
+    def train(self, X, t, epochs=30, batch=256, lr=1e-3, seed=0, order=None):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n = X.shape[0]
+        if order is None:
+            rng = np.random.default_rng(seed)
+            order = np.stack([rng.permutation(n) for _ in range(epochs)])
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        L.check(self.lib.tdf_mlp_train(self.h, L.fptr(X), L.fptr(t), n, L.iptr(order), epochs, batch,
+                                       float(lr), L.fptr(losses)), "tdf_mlp_train")
+        return losses
+
+
+def init_params(seed=0):
+    """He-normal weights, zero biases (flat layout of tdf_mlp_set_params)."""
+    rng = np.random.default_rng(seed)
+    H, I = L.TDF_MLP_HIDDEN, NFEAT
+    parts = [rng.normal(0, math.sqrt(2.0 / I), (H, I)), np.zeros(H),
+             rng.normal(0, math.sqrt(2.0 / H), (H, H)), np.zeros(H),
+             rng.normal(0, math.sqrt(1.0 / H), H), np.zeros(1)]
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+def roc_auc(scores, labels):
+    """P(score of a positive > score of a negative), ties counted half."""
+    s = np.asarray(scores, dtype=np.float64)
+    y = np.asarray(labels).astype(bool)
+    r = np.empty(len(s))
+    order = np.argsort(s, kind="mergesort")
+    ss = s[order]
+    # average ranks over ties
+    starts = np.r_[0, np.nonzero(np.diff(ss))[0] + 1]
+    ends = np.r_[starts[1:], len(s)]
+    for a, b in zip(starts, ends):
+        r[order[a:b]] = (a + b - 1) / 2.0 + 1.0
+    npos, nneg = int(y.sum()), int((~y).sum())
+    return float((r[y].sum() - npos * (npos + 1) / 2.0) / (npos * nneg))
+
+
+# ---------------------------------------------------------------------------
+# training-set generator
+
+def _event_trigger(ctx, sett, opts, f_target, window):
+    """job_core over the spindowns of opts at the current sky point; the
+    strongest trigger with |f - f_target| <= window (rad/sample):
+    (iss, bin, F) or None."""
+    recs = ctx.spindown(0, int((opts.spndr1 - opts.sst) / opts.sstep) + 1, opts)
+    best = None
+    for k, r in enumerate(recs):
+        for f, F in r.ffstat:
+            if abs(f - f_target) <= window and (best is None or F > best[2]):
+                best = (k, float(f), float(F))
+    return best
+
+
+def generate(seg, n_inj, n_lines, rng, snr=(6.0, 20.0), prec=L.TDF_PREC_FP32, window=0.15, log=None):
+    """Runs n_inj CW injections (label 1) and n_lines one-detector
+    stationary lines (label 0), one at a time, through the search of the
+    2-detector segment seg: inject, sky stage at the grid point nearest the
+    source (lines: a random sky point), spindowns ss0-1..ss0+1, strongest
+    trigger within window rad/sample of the injected frequency (the
+    Doppler offset of a line at a wrong sky point reaches ~0.13 rad/sample
+    at 100 Hz; the same window for both classes), its features.  Returns
+    (X [n, 24], labels [n], F [n], meta list)."""
+    sett = seg.sett
+    X, y, Fs, meta = [], [], [], []
+    with TS.Context(sett, seg.dets, prec=prec, batch=8) as ctx:
+        kinds = np.r_[np.ones(n_inj, dtype=int), np.zeros(n_lines, dtype=int)]
+        rng.shuffle(kinds)
+        t0 = time.time()
+        for e, kind in enumerate(kinds):
+            inj = synth.random_injection(rng, float(rng.uniform(*snr)))
+            if kind == 1:
+                ctx.inject(L.TDF_SIGNAL_CW, snr=inj.snr, f=inj.f, fdot=inj.fdot, ra=inj.ra, dec=inj.dec,
+                           iota=inj.iota, psi=inj.psi, phase=inj.phase)
+            else:
+                ctx.inject(L.TDF_SIGNAL_LINE, snr=inj.snr, f=inj.f, phase=inj.phase,
+                           det=int(rng.integers(0, sett.nifo)))
+            pm, mm, nn = synth.sky_to_grid(sett, inj.ra, inj.dec)
+            mm, nn = float(round(mm)), float(round(nn))
+            s0 = round(synth.spindown_to_grid(sett, inj.fdot, mm, nn))
+            opts = TS.default_opts(sett, thr=20.0, sst=s0 - 1, spndr1=s0 + 1, sstep=1)
+            sky = ctx.sky_stage(pm, mm, nn, opts)
+            if not sky.visible:
+                continue
+            tr = _event_trigger(ctx, sett, opts, inj.f, window)
+            if tr is None:
+                continue
+            k, f, F = tr
+            sgnl0 = ctx_sgnl0(sett, sky, mm, nn, opts, k)
+            ic = int(round((f - sgnl0) * sett.nfftf / (2 * math.pi)))
+            _, sums = ctx.trigger_sums(opts, [k], [ic])
+            X.append(features(sums, sky.aa, sky.bb)[0])
+            y.append(kind)
+            Fs.append(F)
+            meta.append(dict(kind=int(kind), snr=inj.snr, f=inj.f, iss=k, ic=ic))
+            if log and (e + 1) % 2000 == 0:
+                log(f"[vetting] {e + 1}/{len(kinds)} events, {len(y)} with a trigger, {time.time() - t0:.1f} s")
+    return (np.array(X, dtype=np.float32).reshape(-1, NFEAT), np.array(y, dtype=np.float32),
+            np.array(Fs), meta)
+
+
+def ctx_sgnl0(sett, sky, mm, nn, opts, iss):
+    """sgnl0 = het0 + het1 of template iss (jobcore.c:430,465-468)."""
+    M = sett.M
+    ss = np.float32(np.float32(sky.smin) + np.float32(iss) * np.float32(opts.sstep))
+    het1 = math.fmod(float(ss) * M[4], M[0])
+    if het1 < 0:
+        het1 += M[0]
+    return sky.het0 + het1
+
+
+def standardize(Xtr, *others):
+    mu = Xtr.mean(0)
+    sd = Xtr.std(0) + 1e-6
+    return [(x - mu) / sd for x in (Xtr,) + others], (mu, sd)
+
+
+def evaluate(n_inj, n_lines, epochs=30, batch=256, lr=1e-3, seed=20261203, holdout=0.2, log=None):
+    """Generate, train on (1-holdout) of the events, report ROC AUC of the
+    network and of F alone on the held-out events."""
+    rng = np.random.default_rng(seed)
+    seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=seed)
+    t0 = time.time()
+    X, y, F, meta = generate(seg, n_inj, n_lines, rng, log=log)
+    tgen = time.time() - t0
+    n = len(y)
+    perm = rng.permutation(n)
+    nte = int(round(holdout * n))
+    te, tr = perm[:nte], perm[nte:]
+    ntr = (len(tr) // batch) * batch
+    tr = tr[:ntr]
+    (Xtr, Xte), _ = standardize(X[tr], X[te])
+    net = Mlp(seed=seed)
+    t1 = time.time()
+    losses = net.train(Xtr, y[tr], epochs=epochs, batch=batch, lr=lr, seed=seed)
+    ttrain = time.time() - t1
+    s_te = net.forward(Xte)
+    auc = roc_auc(s_te, y[te])
+    auc_F = roc_auc(F[te], y[te])
+    return dict(events=int(n_inj + n_lines), samples=int(n), injections_with_trigger=int(y.sum()),
+                lines_with_trigger=int(n - y.sum()), train=int(ntr), heldout=int(nte), epochs=epochs, batch=batch,
+                lr=lr, loss_first=float(losses[0]), loss_last=float(losses[-1]), auc_mlp=auc, auc_F=auc_F,
+                auc_gain=auc - auc_F, generate_s=round(tgen, 2), train_s=round(ttrain, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--injections", type=int, default=20000)
+    ap.add_argument("--lines", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=20261203)
+    a = ap.parse_args()
+    rep = evaluate(a.injections, a.lines, a.epochs, a.batch, a.lr, a.seed,
+                   log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(rep), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,217 @@
+"""GPU parity of the input side and the trigger-vetting path, through the C ABI:
+injection vs the host restatement of add_signal, trigger DFT sums vs the
+oracle and vs the pipeline's own F, the BF16 MFMA network (forward, gradient,
+Adam) vs the numpy oracle, and the end-to-end vetting gain."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import vetting_oracle as V
+from tdfstat_amd import _lib as L
+from tdfstat_amd import search as TS
+from tdfstat_amd import synth
+from tdfstat_amd import vetting
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [(L.TDF_PREC_FP64, 1e-9), (L.TDF_PREC_FP32, 1e-4)]
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def test_mfma_lane_layout():
+    _flush("mfma probe")
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(16, 16)).astype(np.float32)
+    B = rng.normal(size=(16, 16)).astype(np.float32)
+    D = np.zeros((16, 16), dtype=np.float32)
+    L.check(L.load().tdf_mfma_probe(L.fptr(A), L.fptr(B), L.fptr(D)), "tdf_mfma_probe")
+    ref = V.bf16(A) @ V.bf16(B)
+    np.testing.assert_allclose(D, ref, rtol=1e-5, atol=1e-5)
+
+
+def _copy_seg(seg):
+    import copy
+    return copy.deepcopy(seg)
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+@pytest.mark.parametrize("kind", ["cw", "line"])
+def test_inject_matches_host_add_signal(prec, tol, kind):
+    """Device injection (amp mode, h0 given) gives the same per-sky arrays
+    as the host restatement of add_signal applied before upload."""
+    _flush(f"inject {kind} prec={prec}")
+    seg = synth.make_segment(nod=2, N=6000, det_names=("H1", "L1"), seed=21, gap_fracs=(0.1, 0.0))
+    sett = seg.sett
+    inj = synth.Injection(snr=0.0, f=1.3, fdot=-5e-10, ra=2.0, dec=0.4, iota=0.7, psi=0.1, phase=0.5)
+    h0 = 0.37
+    host = _copy_seg(seg)
+    if kind == "cw":
+        sig = synth.signal_template(sett, host.dets, inj)
+    else:
+        i = np.arange(sett.N)
+        sig = [np.cos(inj.f * i + inj.phase), np.zeros(sett.N)]
+    for s, d in zip(sig, host.dets):
+        nz = d.xDat != 0
+        x = d.xDat.astype(np.float64)
+        x[nz] += h0 * s[nz]
+        d.xDat = x.astype(np.float32)
+    opts = TS.default_opts(sett, sst=-1, spndr1=1)
+    with TS.Context(sett, host.dets, prec=prec) as ref, TS.Context(sett, seg.dets, prec=prec) as dev:
+        ref.sky_stage(1, 1.0, 1.0, opts)
+        k = L.TDF_SIGNAL_CW if kind == "cw" else L.TDF_SIGNAL_LINE
+        got_h0 = dev.inject(k, h0=h0, f=inj.f, fdot=inj.fdot, ra=inj.ra, dec=inj.dec, iota=inj.iota,
+                            psi=inj.psi, phase=inj.phase, det=0)
+        assert got_h0 == h0
+        dev.sky_stage(1, 1.0, 1.0, opts)
+        for d in range(2):
+            _, _, xa_r, xb_r = ref.sky_arrays(d)
+            _, _, xa, xb = dev.sky_arrays(d)
+            scale = np.abs(xa_r).max()
+            assert np.abs(xa - xa_r).max() <= max(tol, 1e-6) * scale
+            assert np.abs(xb - xb_r).max() <= max(tol, 1e-6) * np.abs(xb_r).max()
+        # from_clean restores the uploaded data: injecting h0 = 0 gives the clean arrays
+        dev.inject(k, h0=0.0, f=inj.f, det=0)
+        dev.sky_stage(1, 1.0, 1.0, opts)
+        with TS.Context(sett, seg.dets, prec=prec) as clean:
+            clean.sky_stage(1, 1.0, 1.0, opts)
+            np.testing.assert_array_equal(dev.sky_arrays(0)[2], clean.sky_arrays(0)[2])
+
+
+def test_inject_snr_mode():
+    """snr > 0: h0 = snr / sqrt(sum_d sum_{x != 0} s_d^2 / sig2_d)."""
+    _flush("inject snr")
+    seg = synth.make_segment(nod=2, N=6000, det_names=("H1", "L1"), seed=22)
+    sett = seg.sett
+    inj = synth.Injection(snr=9.0, f=0.9, fdot=0.0, ra=1.0, dec=-0.4, iota=0.2, psi=0.3, phase=0.0)
+    sig = synth.signal_template(sett, seg.dets, inj)
+    rho2 = sum(float(np.sum(s[d.xDat != 0] ** 2)) / d.sig2 for s, d in zip(sig, seg.dets))
+    with TS.Context(sett, seg.dets) as ctx:
+        h0 = ctx.inject(L.TDF_SIGNAL_CW, snr=9.0, f=inj.f, fdot=inj.fdot, ra=inj.ra, dec=inj.dec,
+                        iota=inj.iota, psi=inj.psi, phase=inj.phase)
+    assert h0 == pytest.approx(9.0 / math.sqrt(rho2), rel=1e-10)
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_trigger_sums(prec, tol):
+    """GPU direct DFT sums vs the oracle's, and their F vs the pipeline's
+    F-statistic at the same bins."""
+    _flush(f"trigger sums prec={prec}")
+    inj = synth.Injection(snr=30.0, f=1.1, fdot=0.0, ra=0.7, dec=-0.2, iota=0.4, psi=0.3, phase=0.2)
+    seg = synth.make_segment(nod=2, N=6000, det_names=("H1", "L1"), seed=3, injections=[inj])
+    sett = seg.sett
+    opts = TS.default_opts(sett, thr=10.0, sst=-2, spndr1=2, sstep=1)
+    st = O.Oracle(sett, seg.dets).sky_stage(1, 1.0, -1.0, opts)
+    iss, bins = [2, 1, 3], [1100, 2500, 1800]
+    cb_r, s_r = V.trigger_sums(st, sett, 1.0, -1.0, 1, iss, bins)
+    with TS.Context(sett, seg.dets, prec=prec) as ctx:
+        sky = ctx.sky_stage(1, 1.0, -1.0, opts)
+        cb, s = ctx.trigger_sums(opts, iss, bins)
+        np.testing.assert_array_equal(cb, cb_r)
+        scale = np.abs(s_r).max()
+        err = np.abs(s - s_r).max() / scale
+        _flush(f"  sums max err / max |sum| = {err:.3e}")
+        assert err < (1e-10 if prec == L.TDF_PREC_FP64 else 2e-5)
+        ctx.spindown(0, 5, opts)
+        for q in range(3):
+            Fa = s[q, 2, :, :, 0].sum(0)
+            Fb = s[q, 2, :, :, 1].sum(0)
+            F = np.abs(Fa) ** 2 / sky.aa + np.abs(Fb) ** 2 / sky.bb
+            Fp = ctx.fstat(iss[q])[bins[q] - 4: bins[q] + 5]
+            np.testing.assert_allclose(F, Fp, rtol=tol * 10, atol=tol * 10 * Fp.max())
+
+
+def _mlp_data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, V.IN)).astype(np.float32)
+    t = (X[:, 0] + 0.5 * X[:, 3] - X[:, 7] * X[:, 2] > 0).astype(np.float32)
+    return X, t
+
+
+def test_mlp_forward_and_gradient_match_oracle():
+    _flush("mlp forward/gradient")
+    X, t = _mlp_data(1000, 1)
+    w = vetting.init_params(3)
+    net = vetting.Mlp(w)
+    y = net.forward(X)
+    y_ref = V.forward(w.astype(np.float64), X)
+    err = np.abs(y - y_ref).max() / np.abs(y_ref).max()
+    _flush(f"  forward max rel err {err:.3e}")
+    assert err < 2e-3
+    g, loss = net.gradient(X, t)
+    loss_r, g_r, _ = V.loss_and_grad(w.astype(np.float64), X, t)
+    assert loss == pytest.approx(loss_r, rel=1e-4)
+    for name, sl in _slices().items():
+        e = np.abs(g[sl] - g_r[sl]).max() / np.abs(g_r[sl]).max()
+        _flush(f"  grad {name}: max err / max |g| = {e:.3e}")
+        assert e < 5e-3, name
+    # tail tile: n not a multiple of 64
+    g2, _ = net.gradient(X[:77], t[:77])
+    g2_r = V.loss_and_grad(w.astype(np.float64), X[:77], t[:77])[1]
+    assert np.abs(g2 - g2_r).max() / np.abs(g2_r).max() < 5e-3
+
+
+def _slices():
+    out, k = {}, 0
+    for name, shp in V.SHAPES:
+        n = int(np.prod(shp))
+        out[name] = slice(k, k + n)
+        k += n
+    return out
+
+
+def test_mlp_adam_step_matches_oracle():
+    _flush("mlp adam")
+    X, t = _mlp_data(256, 2)
+    w0 = vetting.init_params(4)
+    net = vetting.Mlp(w0)
+    order = np.arange(256, dtype=np.int32)[None, :]
+    net.train(X, t, epochs=1, batch=256, lr=1e-2, order=order)
+    w1 = net.params()
+    g_r = V.loss_and_grad(w0.astype(np.float64), X, t)[1]
+    w_r, _, _ = V.adam(w0.astype(np.float64), 0.0, 0.0, g_r, 1, 1e-2)
+    sel = np.abs(g_r) > 1e-4 * np.abs(g_r).max()
+    np.testing.assert_allclose(w1[sel], w_r[sel], atol=2e-4)
+
+
+def test_mlp_training_learns():
+    _flush("mlp training")
+    X, t = _mlp_data(4096, 5)
+    net = vetting.Mlp(seed=6)
+    losses = net.train(X, t, epochs=20, batch=256, lr=3e-3, seed=7)
+    _flush(f"  loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.5 * losses[0]
+    Xte, tte = _mlp_data(2000, 8)
+    assert vetting.roc_auc(net.forward(Xte), tte) > 0.97
+
+
+def test_vetting_gain_and_threshold_list_unchanged():
+    """Small training set through the 2-day 2-detector search: the network
+    separates injections from single-detector lines better than F alone,
+    and scoring leaves job_core's trigger list unchanged."""
+    _flush("vetting end to end")
+    rep = vetting.evaluate(2500, 2500, epochs=30, batch=128, lr=2e-3, seed=11)
+    _flush(f"  {rep}")
+    assert rep["samples"] > 1500 and rep["lines_with_trigger"] > 150
+    assert rep["auc_mlp"] > 0.95
+    assert rep["auc_mlp"] > rep["auc_F"] + 0.02
+    seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=12,
+                             injections=[synth.random_injection(np.random.default_rng(1), 15.0)])
+    sett = seg.sett
+    opts = TS.default_opts(sett, thr=12.0, sst=-2, spndr1=2)
+    with TS.Context(sett, seg.dets) as ctx:
+        sky, a = ctx.job_core(1, 0.0, 0.0, opts)
+        q = [(k, int(round((f - vetting.ctx_sgnl0(sett, sky, 0.0, 0.0, opts, k)) * sett.nfftf / (2 * math.pi))))
+             for k, r in enumerate(a) for f, _ in r.ffstat]
+        assert q, "no triggers to score"
+        _, sums = ctx.trigger_sums(opts, [k for k, _ in q], [b for _, b in q])
+        scores = vetting.Mlp(seed=1).forward(vetting.features(sums, sky.aa, sky.bb))
+        assert scores.shape == (len(q),) and np.all(np.isfinite(scores))
+        _, b = ctx.job_core(1, 0.0, 0.0, opts)
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.ffstat, y.ffstat)

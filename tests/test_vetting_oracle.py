@@ -1,0 +1,107 @@
+"""Pins of the vetting oracle (CPU): direct DFT sums against the oracle's FFT
+path, backward pass against finite differences, Adam against its closed form,
+ROC AUC against pair counting; the product's feature and AUC host logic."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import vetting_oracle as V
+from tdfstat_amd import search as TS
+from tdfstat_amd import synth
+from tdfstat_amd import vetting
+
+
+def test_trigger_sums_reproduce_fft_fstat():
+    """F = |sum_d Fa_d|^2/aa + |sum_d Fb_d|^2/bb from the direct sums equals the
+    F-statistic of or_spindown (FFT of the zero-padded demodulated series)."""
+    inj = synth.Injection(snr=30.0, f=1.1, fdot=0.0, ra=0.7, dec=-0.2, iota=0.4, psi=0.3, phase=0.2)
+    seg = synth.make_segment(nod=2, N=6000, det_names=("H1", "L1"), seed=3, injections=[inj])
+    sett = seg.sett
+    opts = TS.default_opts(sett, thr=10.0, sst=-2, spndr1=2, sstep=1)
+    orc = O.Oracle(sett, seg.dets)
+    st = orc.sky_stage(1, 1.0, -1.0, opts)
+    iss, bins = [2, 1], [1100, 2500]
+    cb, sums = V.trigger_sums(st, sett, 1.0, -1.0, 1, iss, bins)
+    aa, bb = st["sky"].aa, st["sky"].bb
+    for q in range(2):
+        for o in range(5):
+            Fref = orc.spindown(iss[q] + o - 2, opts)["F"]
+            Fa = sums[q, o, :, :, 0].sum(0)
+            Fb = sums[q, o, :, :, 1].sum(0)
+            F = np.abs(Fa) ** 2 / aa + np.abs(Fb) ** 2 / bb
+            want = Fref[cb[q, o] - 4: cb[q, o] + 5]
+            np.testing.assert_allclose(F, want, rtol=1e-9, atol=1e-9 * want.max())
+        assert cb[q, 2] == bins[q]
+
+
+def _data(n=200, seed=1):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, V.IN))
+    t = (rng.uniform(size=n) < 0.5).astype(float)
+    w = vetting.init_params(seed).astype(np.float64)
+    w[V.NPARAM - 1] = 0.1
+    return w, X, t
+
+
+def test_backward_matches_finite_differences():
+    w, X, t = _data()
+    _, g, _ = V.loss_and_grad(w, X, t, rounded=False)
+    rng = np.random.default_rng(2)
+    idx = np.r_[rng.choice(V.NPARAM, 60, replace=False), V.NPARAM - 1]
+    h = 1e-6
+    for k in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[k] += h
+        wm[k] -= h
+        num = (V.loss_and_grad(wp, X, t, False)[0] - V.loss_and_grad(wm, X, t, False)[0]) / (2 * h)
+        assert abs(num - g[k]) <= 1e-6 + 1e-5 * abs(g[k]), (k, num, g[k])
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -8 + 2 ** -20, -2.5, 0.0, 3.14159265])
+    want = np.array([1.0, 1.0, 1.0 + 4 * 2 ** -8, 1.0 + 2 ** -7, -2.5, 0.0, 3.140625])
+    np.testing.assert_array_equal(V.bf16(x), want)
+
+
+def test_adam_closed_form():
+    """Constant gradient g: m_hat = g, v_hat = g^2, so every step moves
+    w by -lr g / (|g| + eps)."""
+    g = np.array([0.3, -2.0, 1e-3])
+    w = np.zeros(3)
+    m = np.zeros(3)
+    v = np.zeros(3)
+    for step in range(1, 6):
+        w, m, v = V.adam(w, m, v, g, step, 0.01)
+    np.testing.assert_allclose(w, -5 * 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+
+
+def test_roc_auc_pair_count():
+    rng = np.random.default_rng(4)
+    s = np.round(rng.normal(size=300), 1)  # ties
+    y = rng.uniform(size=300) < 0.4
+    pos, neg = s[y], s[~y]
+    brute = (np.sum(pos[:, None] > neg[None, :]) + 0.5 * np.sum(pos[:, None] == neg[None, :])) / (pos.size * neg.size)
+    assert V.roc_auc(s, y) == pytest.approx(brute, abs=1e-12)
+    assert vetting.roc_auc(s, y) == pytest.approx(brute, abs=1e-12)
+    assert vetting.roc_auc(y.astype(float), y) == 1.0
+
+
+def test_features_definition():
+    """A signal in one detector only: balance 1, coherence 1; identical
+    detectors: balance 1/2, coherence 2, phases aligned."""
+    rng = np.random.default_rng(5)
+    base = rng.normal(size=(1, 5, 1, 9, 2)) + 1j * rng.normal(size=(1, 5, 1, 9, 2))
+    one = np.concatenate([base, np.zeros_like(base)], axis=2)
+    f1 = vetting.features(one, 2.0, 3.0)[0]
+    assert f1[19] == pytest.approx(1.0) and f1[20] == pytest.approx(1.0, rel=1e-6)
+    two = np.concatenate([base, base], axis=2)
+    f2 = vetting.features(two, 2.0, 3.0)[0]
+    assert f2[19] == pytest.approx(0.5) and f2[20] == pytest.approx(2.0, rel=1e-6)
+    assert f2[21] == pytest.approx(1.0) and f2[22] == pytest.approx(1.0)
+    Fa = base[0, 2, 0, 4, 0]
+    Fb = base[0, 2, 0, 4, 1]
+    assert f1[0] == pytest.approx(math.log1p(abs(Fa) ** 2 / 2.0), rel=1e-6)
+    assert f1[8] == pytest.approx(math.log1p(abs(Fa) ** 2 / 2.0 + abs(Fb) ** 2 / 3.0), rel=1e-6)
+    assert f1[15] == f1[8]
