@@ -1,8 +1,9 @@
 """Throughput of the TDFstat coherent-search hot path on one MI355X.
 
-A step is one sky point of job_core: the per-sky stage plus its full
-spindown loop (S templates), on a seeded synthetic segment that is already
-resident in HBM.  value = templates/s (sky x spindown) of the FP32-bulk /
+A step is one call of the pipelined multi-sky entry point over a group of
+consecutive sky points of the workload (job_core for each: the per-sky stage
+plus the full spindown loop of S templates), on a seeded synthetic segment
+that is already resident in HBM.  value = templates/s (sky x spindown) of the FP32-bulk /
 FP64-phase path; the plain FP64 path, the FP32-vs-FP64 F-statistic error and
 a CPU baseline (the oracle, reference PREC=FLOAT semantics, OpenMP) are
 measured in the same run.
@@ -27,14 +28,14 @@ HBM_PEAK_GBS = 8000.0
 CONFIGS = {
     # 2 detectors, 2 days (N=86164), nfft=2^17, 31x33 sky x 255 spindowns
     "2day-2det": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(31, 33), sky_step=0.15, nspin=255,
-               injections=4, gaps=None, sigmas=None),
+               injections=4, gaps=None, sigmas=None, per_step=16),
     # 3 detectors, 6 days (N=258492), nfft=2^18, fftpad=2 -> nfftf=2^19,
     # 99x101 sky x 1023 spindowns, 20% gaps
     "6day-3det": dict(dets=("H1", "L1", "V1"), nod=6, fftpad=2, sky=(99, 101), sky_step=0.045, nspin=1023,
-               injections=1, gaps=(0.2, 0.2, 0.2), sigmas=None),
+               injections=1, gaps=(0.2, 0.2, 0.2), sigmas=None, per_step=4),
     # 2 detectors, 2 days, 3x5 sky x 33 spindowns, one injection
     "2day-2det-small": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(3, 5), sky_step=1.0, nspin=33,
-               injections=1, gaps=None, sigmas=None),
+               injections=1, gaps=None, sigmas=None, per_step=15),
 }
 
 
@@ -94,7 +95,9 @@ def kernel_bytes(sett, S, nslots, kname, prec64):
     return None
 
 
-def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, plan=None):
+def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, plan=None, per_step=1):
+    """Times `steps` steps; a step is one tdf_job_core_batch call over
+    per_step consecutive sky points (per_step = 1: one tdf_job_core call)."""
     from tdfstat_amd import search as TS
     sett = seg.sett
     S = cfg["nspin"]
@@ -107,9 +110,21 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
         p = ctx.autotune()
         print(f"[bench] prec={prec} autotuned plan {p}", file=sys.stderr, flush=True)
     oc = opts.to_c()
-    cap = (S + 1) * 64
+    cap = (S + 1) * 64 * per_step
+    npts = len(pts)
+
+    def step(k):
+        if per_step == 1:
+            sky, nrec, npairs = ctx.job_core_raw(*pts[k % npts], oc, S + 1, cap)
+            return nrec, npairs
+        idx = [(k * per_step + j) % npts for j in range(per_step)]
+        pm = np.array([pts[i][0] for i in idx], dtype=np.int32)
+        mm = np.array([pts[i][1] for i in idx], dtype=np.float32)
+        nn = np.array([pts[i][2] for i in idx], dtype=np.float32)
+        return ctx.job_core_batch_raw(pm, mm, nn, oc, (S + 1) * per_step, cap)
+
     for k in range(warmup):
-        ctx.job_core_raw(*pts[k % len(pts)], oc, S + 1, cap)
+        step(k)
     ctx.synchronize()
     if profile:
         ctx.profile(True)
@@ -119,7 +134,7 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
     nt = 0
     ntrig = 0
     for k in range(steps):
-        sky, nrec, npairs = ctx.job_core_raw(*pts[(warmup + k) % len(pts)], oc, S + 1, cap)
+        nrec, npairs = step(warmup + k)
         nt += nrec
         ntrig += npairs
     ctx.synchronize()
@@ -236,6 +251,9 @@ def main():
     ap.add_argument("--plan", default=None,
                     help="FP32 plan n1,e1,e2,fused,batch instead of tdf_autotune (profiling runs)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
+    ap.add_argument("--per-step", type=int, default=0,
+                    help="sky points per step (one tdf_job_core_batch call); 1 = one tdf_job_core call per step; "
+                         "0 = the workload's default")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -265,15 +283,16 @@ def main():
     plan = tuple(int(v) for v in args.plan.split(",")) if args.plan else None
     if plan is not None and len(plan) != 5:
         ap.error("--plan takes n1,e1,e2,fused,batch")
+    per_step = args.per_step or cfg["per_step"]
     ctx32, opts, nt, el, kt, ntrig = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
-                                             args.batch, not args.no_tune, plan)
+                                             args.batch, not args.no_tune, plan, per_step)
     value = nt / el
     print(f"[bench] fp32: {nt} templates in {el:.3f} s -> {value:.0f} templates/s", file=sys.stderr, flush=True)
     fp64 = fperr = None
     if not args.no_fp64:
         k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 8)
         ctx64, _, nt64, el64, kt64, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
-                                                not args.no_tune)
+                                                not args.no_tune, None, per_step)
         value64 = nt64 / el64
         print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
               flush=True)
@@ -309,7 +328,8 @@ def main():
         "config": {"workload": f"{args.config}: nifo={sett.nifo} N={sett.N} nfft={sett.nfft} "
                                f"nfftf={sett.nfftf} {len(pts)} sky x {S} spindowns, seeded Gaussian "
                                f"noise + injections, FP64 phase",
-                   "templates_timed": nt, "triggers": ntrig, "plan": line_plan},
+                   "templates_timed": nt, "triggers": ntrig, "plan": line_plan,
+                   "sky_points_per_step": per_step},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "avg_launch_ms": round(ms_tot / launches, 5),

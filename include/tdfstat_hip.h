@@ -137,6 +137,19 @@ int tdf_job_core(tdf_ctx *ctx, int pm, float mm, float nn,
                  int rec_cap, float *pairs, long long pair_cap, int *nrec,
                  long long *npairs);
 
+/* The sky loop of search() (jobcore.c:88-111) over nsky points
+ * (pm[k], mm[k], nn[k]) in one call: job_core for each, pipelined on two
+ * streams (the per-sky stage of a point runs while the spindown loop of the
+ * previous point runs; trigger read-back overlaps the next point's work).
+ * skies[k] receive each point's scalars and nrec_per[k] its record count
+ * (iss_size, 0 if not visible); records are written consecutively in input
+ * order, (f, F) pairs into pairs.  *nrec / *npairs return the totals.  The
+ * records and pairs equal those of nsky tdf_job_core calls. */
+int tdf_job_core_batch(tdf_ctx *ctx, int nsky, const int *pm, const float *mm, const float *nn,
+                       const tdf_job_opts *o, tdf_sky *skies, int *nrec_per, tdf_record *recs,
+                       long long rec_cap, float *pairs, long long pair_cap, long long *nrec,
+                       long long *npairs);
+
 /* ---- inspection (tests) ---- */
 /* per-sky arrays of detector det, converted to double: tmp1[N],
  * shftf[N], xDatma/xDatmb[2N] interleaved complex (after /sig2).

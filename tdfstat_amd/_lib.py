@@ -89,6 +89,9 @@ SIGNATURES = {
                                C.POINTER(TdfSky), C.POINTER(TdfRecord), C.c_int,
                                C.POINTER(C.c_float), C.c_longlong, C.POINTER(C.c_int),
                                C.POINTER(C.c_longlong)]),
+    "tdf_job_core_batch": (C.c_int, [_P, C.c_int, _IP, _FP, _FP, C.POINTER(TdfJobOpts), C.POINTER(TdfSky), _IP,
+                                     C.POINTER(TdfRecord), C.c_longlong, _FP, C.c_longlong,
+                                     C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "tdf_debug_sky_arrays": (C.c_int, [_P, C.c_int, _DP, _DP, _DP, _DP]),
     "tdf_debug_fstat": (C.c_int, [_P, C.c_int, _DP]),
     "tdf_debug_sky_interp": (C.c_int, [_P, C.c_int, _DP, _DP]),

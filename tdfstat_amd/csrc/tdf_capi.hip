@@ -107,6 +107,16 @@ struct tdf_ctx {
     std::vector<double> lines_h;
     int last_batch_n = 0, last_batch_iss0 = 0, call_iss0 = 0;
     int last_fr0 = 0, last_fr1 = -1, last_nblk = 0, last_nav = 1;
+    // multi-sky pipeline (tdf_job_core_batch): second per-sky slot, sky
+    // stream, events, double-buffered pinned readback
+    hipStream_t sstream = nullptr;
+    void *xm_s[2] = {nullptr, nullptr};
+    double *tmp1_s[2] = {nullptr, nullptr}, *scal_s[2] = {nullptr, nullptr};
+    hipEvent_t evS[2] = {nullptr, nullptr}, evA[2] = {nullptr, nullptr};
+    int *cnt_hb[2] = {nullptr, nullptr};
+    TrigEntry *trig_hb[2] = {nullptr, nullptr};
+    double *scal_hb[2] = {nullptr, nullptr};
+    int cnt_hb_cap = 0;
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev;
@@ -143,7 +153,8 @@ struct Prof {
     tdf_ctx *c;
     int slot;
     size_t idx;
-    Prof(tdf_ctx *c_, int s) : c(c_), slot(s), idx(0)
+    hipStream_t st;
+    Prof(tdf_ctx *c_, int s, hipStream_t st_ = nullptr) : c(c_), slot(s), idx(0), st(st_ ? st_ : c_->stream)
     {
         if (!c->prof) return;
         if (c->ev_used + 2 > c->ev.size()) {
@@ -157,12 +168,12 @@ struct Prof {
         }
         idx = c->ev_used;
         c->ev_slot[idx / 2 * 2] = s;
-        (void)hipEventRecord(c->ev[idx], c->stream);
+        (void)hipEventRecord(c->ev[idx], st);
     }
     ~Prof()
     {
         if (!c->prof) return;
-        (void)hipEventRecord(c->ev[idx + 1], c->stream);
+        (void)hipEventRecord(c->ev[idx + 1], st);
         c->ev_used += 2;
     }
 };
@@ -248,51 +259,50 @@ int make_plan(tdf_ctx *c, int n, FftPlan *p, int n1_req = 0, int e1 = 0, int e2 
 SpinLayout spin_layout(const tdf_ctx *c) { return SpinLayout{c->p_spin.n1, c->p_spin.l1, c->p_spin.n2}; }
 
 template <typename T>
-int run_sky_stage(tdf_ctx *c, const SkyPrepArgs &a)
+int run_sky_stage(tdf_ctx *c, const SkyPrepArgs &a, hipStream_t st)
 {
     const tdf_geom &g = c->g;
     const int nifo = g.nifo;
-    hipStream_t st = c->stream;
     {
-        Prof pr(c, KS_SKY_PREP);
+        Prof pr(c, KS_SKY_PREP, st);
         HIPCHK(launch_sky_prep<T>(a, c->xab, c->prep_nblk, nifo, st));
     }
     {
-        Prof pr(c, KS_SKY_REDUCE);
+        Prof pr(c, KS_SKY_REDUCE, st);
         HIPCHK(launch_sky_reduce(c->partials, c->prep_nblk, nifo, c->sig2_d, c->sky_scal, st));
     }
     // forward FFT of length nfft, result column-major in XT
     {
-        Prof pr(c, KS_SKY_FFT);
+        Prof pr(c, KS_SKY_FFT, st);
         HIPCHK(launch_fft_natural_to_T<T>(c->p_fwd, c->xab, c->Zsky, c->XT, nifo, -1, 0, 0, 1.0, st));
     }
     // natural order + spectrum move into Ymv (jobcore.c:340-350)
     const int nyqst = g.nfft / 2 + 1;
     {
-        Prof pr(c, KS_SKY_TRANSPOSE);
+        Prof pr(c, KS_SKY_TRANSPOSE, st);
         HIPCHK(launch_transpose<T>(c->p_fwd, c->XT, c->Ymv, 2 * nifo, (size_t)g.nfft, (size_t)g.Ninterp, nyqst,
                                    g.Ninterp - g.nfft, g.nfft, st));
     }
     // inverse FFT of length Ninterp, scaled by interpftpad/Ninterp
     const double ft = (double)g.interpftpad / g.Ninterp;
     {
-        Prof pr(c, KS_SKY_IFFT);
+        Prof pr(c, KS_SKY_IFFT, st);
         HIPCHK(launch_fft_natural_to_T<T>(c->p_inv, c->Ymv, c->Zsky, c->XT, nifo, +1, nyqst,
                                           nyqst + g.Ninterp - g.nfft, ft, st));
     }
     const int nnodes = g.interpftpad * g.N;
     {
-        Prof pr(c, KS_SKY_TRANSPOSE2);
+        Prof pr(c, KS_SKY_TRANSPOSE2, st);
         HIPCHK(launch_transpose<T>(c->p_inv, c->XT, c->Ymv, 2 * nifo, (size_t)g.Ninterp, (size_t)g.Ninterp,
                                    g.Ninterp, 0, std::min(g.Ninterp, nnodes + 1), st));
     }
     {
-        Prof pr(c, KS_SPLINE);
+        Prof pr(c, KS_SPLINE, st);
         HIPCHK(launch_spline<T>(c->Ymv, c->Zsky, nnodes, (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, c->taps,
                                 st));
     }
     {
-        Prof pr(c, KS_SPLINT);
+        Prof pr(c, KS_SPLINT, st);
         HIPCHK(launch_splint<T>(c->Ymv, c->Zsky, c->shftf, c->sig2_d, c->xm, g.N, nnodes, g.interpftpad,
                                 (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, spin_layout(c),
                                 (size_t)g.nfftf, st));
@@ -399,6 +409,17 @@ int ensure_counters(tdf_ctx *c, int n)
     const int cap = std::max(need, 2 * c->cnt_cap);
     if (c->cnt_d) (void)hipFree(c->cnt_d);
     if (c->cnt_h) (void)hipHostFree(c->cnt_h);
+    for (int k = 0; k < 2; k++) {
+        if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
+        if (c->trig_hb[k]) (void)hipHostFree(c->trig_hb[k]);
+        if (c->scal_hb[k]) (void)hipHostFree(c->scal_hb[k]);
+        if (c->evS[k]) (void)hipEventDestroy(c->evS[k]);
+        if (c->evA[k]) (void)hipEventDestroy(c->evA[k]);
+    }
+    if (c->sstream) {
+        (void)hipStreamSynchronize(c->sstream);
+        (void)hipStreamDestroy(c->sstream);
+    }
     c->cnt_d = nullptr;
     c->cnt_h = nullptr;
     c->cnt_cap = 0;
@@ -406,6 +427,75 @@ int ensure_counters(tdf_ctx *c, int n)
     if (hipHostMalloc((void **)&c->cnt_h, sizeof(int) * cap, 0) != hipSuccess) return TDF_ERR_NOMEM;
     c->cnt_cap = cap;
     return TDF_OK;
+}
+
+// Host part of the sky stage (jobcore.c:242-262, 399-411): visibility,
+// lin2ast, het0, the spindown range and the per-sky kernel arguments.  aa
+// and bb are left to the device.
+void sky_setup(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o, tdf_sky *sp, SkyPrepArgs *ap)
+{
+    tdf_sky &s = *sp;
+    SkyPrepArgs &a = *ap;
+    s = tdf_sky{};
+    const tdf_geom &g = c->g;
+    const double *M = g.M;
+    // jobcore.c:242-248
+    const double al1 = nn * M[10] + mm * M[14];
+    const double al2 = nn * M[11] + mm * M[15];
+    const double cof = g.oms;
+    if ((al1 * al1 + al2 * al2) / (cof * cof) > 1.) {
+        s.visible = 0;
+        return;
+    }
+    s.visible = 1;
+    // lin2ast (auxi.c:22-29)
+    const double be1 = al1 / cof, be2 = al2 / cof;
+    const double sindel = be1 * g.sepsm - (2 * pm - 3) * sqrt(1. - be1 * be1 - be2 * be2) * g.cepsm;
+    const double cosdel = sqrt(1. - sindel * sindel);
+    const double sinal = (be1 - g.sepsm * sindel) / (g.cepsm * cosdel);
+    const double cosal = be2 / cosdel;
+    s.dec = asin(sindel);
+    s.ra = fmod(atan2(sinal, cosal) + 2. * M_PI, 2. * M_PI);
+    s.het0 = fmod(nn * M[8] + mm * M[12], M[0]);
+    a.lay = spin_layout(c);
+    a.xDat = c->xDat;
+    a.DetSSB = c->DetSSB;
+    a.det = c->detp;
+    a.shftf = c->shftf;
+    a.tmp1 = c->tmp1;  // current slot
+    a.partials = c->partials;
+    a.N = g.N;
+    a.nfft = g.nfft;
+    a.omr = g.omr;
+    a.oms = g.oms;
+    a.het0 = s.het0;
+    a.ns0 = cosal * cosdel;
+    a.ns1 = sinal * cosdel;
+    a.ns2 = sindel;
+    a.c2d = cosdel * cosdel;
+    a.c2sd = sindel * cosdel;
+    a.sindel = sindel;
+    a.cosdel = cosdel;
+    for (int d = 0; d < g.nifo; d++) {
+        const DetParams &dp = c->detp_h[d];
+        a.cosalfr[d] = cosal * dp.cphir + sinal * dp.sphir;
+        a.sinalfr[d] = sinal * dp.cphir - cosal * dp.sphir;
+    }
+    // spindown range (jobcore.c:399-411)
+    float smin = o->sst, smax = o->spndr1;
+    if (o->gtype_allsky) {
+        smin = trunc((g.Smin - nn * M[9] - mm * M[13]) / M[5]);
+        smax = trunc(-(nn * M[9] + mm * M[13] + g.Smax) / M[5]);
+        if (smin > smax) {
+            smin = smin + smax;
+            smax = smin - smax;
+            smin = smin - smax;
+        }
+    }
+    s.smin = smin;
+    s.smax = smax;
+    s.iss_size = (int)((smax - smin) / o->sstep + 1);
+    if (s.iss_size < 0) s.iss_size = 0;
 }
 
 }  // namespace
@@ -571,56 +661,16 @@ int tdf_sky_stage(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o,
     if (rc) return rc;
     for (int d = 0; d < c->g.nifo; d++)
         if (!c->det_set[d]) return TDF_ERR_STATE;
-    const tdf_geom &g = c->g;
-    const double *M = g.M;
-    tdf_sky s{};
     c->sky_valid = false;
-    // jobcore.c:242-248
-    const double al1 = nn * M[10] + mm * M[14];
-    const double al2 = nn * M[11] + mm * M[15];
-    const double cof = g.oms;
-    if ((al1 * al1 + al2 * al2) / (cof * cof) > 1.) {
-        s.visible = 0;
+    tdf_sky s{};
+    SkyPrepArgs a{};
+    sky_setup(c, pm, mm, nn, o, &s, &a);
+    if (!s.visible) {
         *sky = s;
         c->sky = s;
         return TDF_OK;
     }
-    s.visible = 1;
-    // lin2ast (auxi.c:22-29)
-    const double be1 = al1 / cof, be2 = al2 / cof;
-    const double sindel = be1 * g.sepsm - (2 * pm - 3) * sqrt(1. - be1 * be1 - be2 * be2) * g.cepsm;
-    const double cosdel = sqrt(1. - sindel * sindel);
-    const double sinal = (be1 - g.sepsm * sindel) / (g.cepsm * cosdel);
-    const double cosal = be2 / cosdel;
-    s.dec = asin(sindel);
-    s.ra = fmod(atan2(sinal, cosal) + 2. * M_PI, 2. * M_PI);
-    s.het0 = fmod(nn * M[8] + mm * M[12], M[0]);
-    SkyPrepArgs a{};
-    a.lay = spin_layout(c);
-    a.xDat = c->xDat;
-    a.DetSSB = c->DetSSB;
-    a.det = c->detp;
-    a.shftf = c->shftf;
-    a.tmp1 = c->tmp1;
-    a.partials = c->partials;
-    a.N = g.N;
-    a.nfft = g.nfft;
-    a.omr = g.omr;
-    a.oms = g.oms;
-    a.het0 = s.het0;
-    a.ns0 = cosal * cosdel;
-    a.ns1 = sinal * cosdel;
-    a.ns2 = sindel;
-    a.c2d = cosdel * cosdel;
-    a.c2sd = sindel * cosdel;
-    a.sindel = sindel;
-    a.cosdel = cosdel;
-    for (int d = 0; d < g.nifo; d++) {
-        const DetParams &dp = c->detp_h[d];
-        a.cosalfr[d] = cosal * dp.cphir + sinal * dp.sphir;
-        a.sinalfr[d] = sinal * dp.cphir - cosal * dp.sphir;
-    }
-    rc = c->prec == TDF_PREC_FP64 ? run_sky_stage<double>(c, a) : run_sky_stage<float>(c, a);
+    rc = c->prec == TDF_PREC_FP64 ? run_sky_stage<double>(c, a, c->stream) : run_sky_stage<float>(c, a, c->stream);
     if (rc) return rc;
     double scal[2];
     HIPCHK(hipMemcpyAsync(scal, c->sky_scal, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
@@ -628,21 +678,6 @@ int tdf_sky_stage(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o,
     prof_collect(c);
     s.aa = scal[0];
     s.bb = scal[1];
-    // spindown range (jobcore.c:399-411)
-    float smin = o->sst, smax = o->spndr1;
-    if (o->gtype_allsky) {
-        smin = trunc((g.Smin - nn * M[9] - mm * M[13]) / M[5]);
-        smax = trunc(-(nn * M[9] + mm * M[13] + g.Smax) / M[5]);
-        if (smin > smax) {
-            smin = smin + smax;
-            smax = smin - smax;
-            smin = smin - smax;
-        }
-    }
-    s.smin = smin;
-    s.smax = smax;
-    s.iss_size = (int)((smax - smin) / o->sstep + 1);
-    if (s.iss_size < 0) s.iss_size = 0;
     *sky = s;
     c->sky = s;
     c->pm = pm;
@@ -763,6 +798,245 @@ int tdf_spindown(tdf_ctx *c, int iss_begin, int iss_count, const tdf_job_opts *o
     }
     if (npairs) *npairs = np;
     return TDF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int BATCH_TRIG_PREFIX = 1 << 16;  // triggers read back per sky point without a sync
+
+// Second per-sky slot, sky stream, events and pinned readback buffers of
+// the multi-sky pipeline; allocated on first use.
+int ensure_batch(tdf_ctx *c, int ncnt)
+{
+    const tdf_geom &g = c->g;
+    if (!c->sstream) {
+        HIPCHK(hipStreamCreateWithFlags(&c->sstream, hipStreamNonBlocking));
+        c->xm_s[0] = c->xm;
+        c->tmp1_s[0] = c->tmp1;
+        c->scal_s[0] = c->sky_scal;
+        int rc;
+        if ((rc = dmalloc(c, &c->xm_s[1], c->csz * 2 * g.nifo * (size_t)g.nfftf))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->tmp1_s[1], sizeof(double) * g.nifo * (size_t)g.nfftf))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->scal_s[1], sizeof(double) * 4))) return rc;
+        HIPCHK(hipMemset(c->xm_s[1], 0, c->csz * 2 * g.nifo * (size_t)g.nfftf));
+        HIPCHK(hipMemset(c->tmp1_s[1], 0, sizeof(double) * g.nifo * (size_t)g.nfftf));
+        for (int k = 0; k < 2; k++) {
+            HIPCHK(hipEventCreateWithFlags(&c->evS[k], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&c->evA[k], hipEventDisableTiming));
+            // recorded once so that the first waits on them are satisfied
+            HIPCHK(hipEventRecord(c->evS[k], c->sstream));
+            HIPCHK(hipEventRecord(c->evA[k], c->stream));
+            if (hipHostMalloc((void **)&c->trig_hb[k], sizeof(TrigEntry) * BATCH_TRIG_PREFIX, 0) != hipSuccess ||
+                hipHostMalloc((void **)&c->scal_hb[k], sizeof(double) * 4, 0) != hipSuccess)
+                return TDF_ERR_NOMEM;
+        }
+    }
+    if (ncnt > c->cnt_hb_cap) {
+        for (int k = 0; k < 2; k++) {
+            if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
+            c->cnt_hb[k] = nullptr;
+            if (hipHostMalloc((void **)&c->cnt_hb[k], sizeof(int) * ncnt, 0) != hipSuccess) return TDF_ERR_NOMEM;
+        }
+        c->cnt_hb_cap = ncnt;
+    }
+    return TDF_OK;
+}
+
+void use_slot(tdf_ctx *c, int slot)
+{
+    c->xm = c->xm_s[slot];
+    c->tmp1 = c->tmp1_s[slot];
+    c->sky_scal = c->scal_s[slot];
+}
+
+// Records of one spindown call from its trigger list (ascending bin per
+// record, jobcore.c:437-456 for the record fields).
+int fill_records(tdf_ctx *c, const tdf_sky &sky, float mm, float nn, const tdf_job_opts *o, int iss_count,
+                 const TrigEntry *tr, long long ntot, tdf_record *recs, float *pairs, long long pair_cap,
+                 long long *np)
+{
+    const double *M = c->g.M;
+    const SkyScal sk{sky.smin, o->sstep, mm, nn, sky.het0, M[0], M[4], M[5], M[9], M[13], 0};
+    std::vector<std::vector<std::pair<int, std::pair<float, float>>>> per(iss_count);
+    for (long long t = 0; t < ntot; t++) per[tr[t].slot].push_back({tr[t].ic, {tr[t].f, tr[t].F}});
+    for (int k = 0; k < iss_count; k++) {
+        const TmplDesc td = tmpl_scalars(sk, k);
+        tdf_record &r = recs[k];
+        r.m = mm;
+        r.n = nn;
+        r.s = sky.smin + (float)k * o->sstep;
+        r.ra = (float)sky.ra;
+        r.dec = (float)sky.dec;
+        r.fdot = (float)(td.spnd / (M_PI * c->g.dt * c->g.dt));
+        auto &v = per[k];
+        std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        r.ntrig = (int)v.size();
+        r.offset = *np;
+        if (*np + (long long)v.size() > pair_cap) return TDF_ERR_CAPACITY;
+        for (auto &e : v) {
+            pairs[2 * *np] = e.second.first;
+            pairs[2 * *np + 1] = e.second.second;
+            (*np)++;
+        }
+    }
+    return TDF_OK;
+}
+
+template <typename T>
+int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const float *nn, const tdf_job_opts *o,
+                   tdf_sky *skies, int *nrec_per, tdf_record *recs, long long rec_cap, float *pairs,
+                   long long pair_cap, long long *nrec, long long *npairs)
+{
+    const tdf_geom &g = c->g;
+    const double *M = g.M;
+    std::vector<tdf_sky> S(nsky);
+    std::vector<SkyPrepArgs> A(nsky);
+    std::vector<long long> roff(nsky, 0);
+    std::vector<int> vis;
+    long long rtot = 0;
+    int maxiss = 1;
+    for (int k = 0; k < nsky; k++) {
+        sky_setup(c, pm[k], mm[k], nn[k], o, &S[k], &A[k]);
+        nrec_per[k] = 0;
+        roff[k] = rtot;
+        if (S[k].visible) {
+            vis.push_back(k);
+            rtot += std::max(S[k].iss_size, 0);
+            maxiss = std::max(maxiss, S[k].iss_size);
+        }
+        skies[k] = S[k];
+    }
+    if (rtot > rec_cap) return TDF_ERR_CAPACITY;
+    int rc;
+    if ((rc = ensure_counters(c, maxiss))) return rc;
+    const int ncnt = 1 + maxiss + (maxiss + c->batch - 1) / c->batch;
+    if ((rc = ensure_batch(c, ncnt))) return rc;
+    if (o->numlines > 0 && (o->numlines != c->lines_n ||
+                            memcmp(c->lines_h.data(), o->lines, sizeof(double) * 2 * o->numlines) != 0)) {
+        c->lines_h.assign(o->lines, o->lines + 2 * o->numlines);
+        c->lines_n = o->numlines;
+        HIPCHK(hipMemcpyAsync(c->lines_d, c->lines_h.data(), sizeof(double) * 2 * o->numlines,
+                              hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const bool use_cand = !o->fstat_norm || (o->fr1 - o->fr0) / std::max(o->nav, 1) == 0;
+    const int maxtrig = o->bufsize / 2;
+    const long long prefix = std::min<long long>(c->trig_cap, BATCH_TRIG_PREFIX);
+    std::vector<int> redo;
+    bool overflow = false;
+    long long np = 0;
+    c->call_iss0 = 0;
+
+    auto enqueue_sky = [&](int j, int slot) -> int {
+        use_slot(c, slot);
+        A[j].tmp1 = c->tmp1;
+        HIPCHK(hipStreamWaitEvent(c->sstream, c->evA[slot], 0));
+        int r = run_sky_stage<T>(c, A[j], c->sstream);
+        if (r) return r;
+        HIPCHK(hipMemcpyAsync(c->scal_hb[slot], c->scal_s[slot], sizeof(double) * 2, hipMemcpyDeviceToHost,
+                              c->sstream));
+        HIPCHK(hipEventRecord(c->evS[slot], c->sstream));
+        return TDF_OK;
+    };
+    auto enqueue_spin = [&](int j, int slot) -> int {
+        use_slot(c, slot);
+        HIPCHK(hipStreamWaitEvent(c->stream, c->evS[slot], 0));
+        const int iss = S[j].iss_size;
+        if (iss > 0) {
+            const SkyScal sk{S[j].smin, o->sstep, mm[j], nn[j], S[j].het0, M[0], M[4], M[5], M[9], M[13], 0};
+            const int nc = 1 + iss + (iss + c->batch - 1) / c->batch;
+            HIPCHK(hipMemsetAsync(c->cnt_d, 0, sizeof(int) * nc, c->stream));
+            int r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_d, iss);
+            if (r) return r;
+            HIPCHK(hipMemcpyAsync(c->cnt_hb[slot], c->cnt_d, sizeof(int) * nc, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(c->trig_hb[slot], c->trig_d, sizeof(TrigEntry) * prefix, hipMemcpyDeviceToHost,
+                                  c->stream));
+        }
+        HIPCHK(hipEventRecord(c->evA[slot], c->stream));
+        return TDF_OK;
+    };
+    auto finish = [&](int j, int slot) -> int {
+        HIPCHK(hipEventSynchronize(c->evA[slot]));
+        S[j].aa = c->scal_hb[slot][0];
+        S[j].bb = c->scal_hb[slot][1];
+        skies[j] = S[j];
+        const int iss = S[j].iss_size;
+        if (iss <= 0) return TDF_OK;
+        const int *cnt = c->cnt_hb[slot];
+        for (int k = 0; k < iss; k++)
+            if (cnt[1 + k] > maxtrig) overflow = true;
+        bool again = false;
+        if (use_cand)
+            for (int b = 0; b < (iss + c->batch - 1) / c->batch; b++)
+                if (cnt[1 + iss + b] > c->cand_cap) again = true;
+        const long long ntot = cnt[0];
+        if (ntot > prefix) again = true;
+        if (again || overflow) {
+            if (!overflow) redo.push_back(j);
+            return TDF_OK;
+        }
+        nrec_per[j] = iss;
+        return fill_records(c, S[j], mm[j], nn[j], o, iss, c->trig_hb[slot], ntot, recs + roff[j], pairs,
+                            pair_cap, &np);
+    };
+
+    const int n = (int)vis.size();
+    for (int idx = 0; idx <= n + 1 && !rc && !overflow; idx++) {
+        if (idx >= 1 && idx <= n) rc = enqueue_spin(vis[idx - 1], (idx - 1) % 2);
+        if (!rc && idx >= 2) rc = finish(vis[idx - 2], idx % 2);
+        if (!rc && idx < n) rc = enqueue_sky(vis[idx], idx % 2);
+    }
+    (void)hipStreamSynchronize(c->sstream);
+    (void)hipStreamSynchronize(c->stream);
+    use_slot(c, 0);
+    c->sky_valid = false;
+    prof_collect(c);
+    if (rc) return rc;
+    if (overflow) return TDF_ERR_TRIGGER_OVERFLOW;
+    // points whose triggers did not fit the pipelined readback: synchronous path
+    for (int j : redo) {
+        tdf_sky sj;
+        if ((rc = tdf_sky_stage(c, pm[j], mm[j], nn[j], o, &sj))) return rc;
+        long long npj = 0;
+        if ((rc = tdf_spindown(c, 0, sj.iss_size, o, recs + roff[j], pairs + 2 * np, pair_cap - np, &npj)))
+            return rc;
+        for (int k = 0; k < sj.iss_size; k++) recs[roff[j] + k].offset += np;
+        np += npj;
+        nrec_per[j] = sj.iss_size;
+        skies[j] = sj;
+    }
+    *nrec = rtot;
+    if (npairs) *npairs = np;
+    return TDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdf_job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const float *nn, const tdf_job_opts *o,
+                       tdf_sky *skies, int *nrec_per, tdf_record *recs, long long rec_cap, float *pairs,
+                       long long pair_cap, long long *nrec, long long *npairs)
+{
+    if (!c || nsky < 0 || !nrec || (nsky > 0 && (!pm || !mm || !nn || !skies || !nrec_per || !recs)) ||
+        (pair_cap > 0 && !pairs))
+        return TDF_ERR_ARG;
+    *nrec = 0;
+    if (npairs) *npairs = 0;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    for (int d = 0; d < c->g.nifo; d++)
+        if (!c->det_set[d]) return TDF_ERR_STATE;
+    if (o->numlines > c->lines_cap) return TDF_ERR_UNSUPPORTED;
+    if (o->fstat_norm && (o->fr1 - o->fr0) / o->nav > c->nblk_cap) return TDF_ERR_UNSUPPORTED;
+    if (nsky == 0) return TDF_OK;
+    return c->prec == TDF_PREC_FP64
+               ? job_core_batch<double>(c, nsky, pm, mm, nn, o, skies, nrec_per, recs, rec_cap, pairs, pair_cap,
+                                        nrec, npairs)
+               : job_core_batch<float>(c, nsky, pm, mm, nn, o, skies, nrec_per, recs, rec_cap, pairs, pair_cap,
+                                       nrec, npairs);
 }
 
 int tdf_job_core(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o, tdf_sky *sky, tdf_record *recs,

@@ -150,6 +150,49 @@ class Context:
                 "tdf_job_core")
         return sky, _records(recs, nrec.value, pairs)
 
+    def job_core_batch(self, points, opts, max_records=None, pair_cap=None):
+        """job_core for a list of sky points (pm, mm, nn) in one pipelined
+        call -> (skies, records per point, [Record])."""
+        pts = list(points)
+        n = len(pts)
+        pm = np.array([p[0] for p in pts], dtype=np.int32)
+        mm = np.array([p[1] for p in pts], dtype=np.float32)
+        nn = np.array([p[2] for p in pts], dtype=np.float32)
+        if max_records is None:
+            max_records = n * (int((opts.spndr1 - opts.sst) / opts.sstep) + 1 if not opts.gtype_allsky else 4096)
+        if pair_cap is None:
+            pair_cap = max(1024, max_records * 64)
+        o = opts.to_c()
+        skies = (L.TdfSky * max(1, n))()
+        nper = np.zeros(max(1, n), dtype=np.int32)
+        recs = (L.TdfRecord * max(1, max_records))()
+        pairs = np.zeros(2 * pair_cap, dtype=np.float32)
+        nrec = C.c_longlong(0)
+        npairs = C.c_longlong(0)
+        L.check(self.lib.tdf_job_core_batch(self.h, n, L.iptr(pm), L.fptr(mm), L.fptr(nn), C.byref(o), skies,
+                                            L.iptr(nper), recs, max_records, L.fptr(pairs), pair_cap,
+                                            C.byref(nrec), C.byref(npairs)), "tdf_job_core_batch")
+        return [skies[k] for k in range(n)], nper[:n].copy(), _records(recs, nrec.value, pairs)
+
+    def job_core_batch_raw(self, pm, mm, nn, opts_c, max_records, pair_cap):
+        """tdf_job_core_batch into buffers owned by the context (numpy int32
+        pm, float32 mm, nn); returns (nrec, npairs)."""
+        n = pm.size
+        key = (n, max_records, pair_cap)
+        if getattr(self, "_bkey", None) != key:
+            self._b_skies = (L.TdfSky * max(1, n))()
+            self._b_nper = np.zeros(max(1, n), dtype=np.int32)
+            self._b_recs = (L.TdfRecord * max(1, max_records))()
+            self._b_pairs = np.zeros(2 * pair_cap, dtype=np.float32)
+            self._bkey = key
+        nrec = C.c_longlong(0)
+        npairs = C.c_longlong(0)
+        L.check(self.lib.tdf_job_core_batch(self.h, n, L.iptr(pm), L.fptr(mm), L.fptr(nn), C.byref(opts_c),
+                                            self._b_skies, L.iptr(self._b_nper), self._b_recs, max_records,
+                                            L.fptr(self._b_pairs), pair_cap, C.byref(nrec), C.byref(npairs)),
+                "tdf_job_core_batch")
+        return nrec.value, npairs.value
+
     def job_core_raw(self, pm, mm, nn, opts_c, max_spindowns, pair_cap):
         """job_core into buffers owned by the context; returns
         (sky, nrec, npairs).  opts_c comes from JobOpts.to_c()."""
