@@ -4,7 +4,9 @@ scipy's natural CubicSpline), the published closed form of the amplitude
 modulation (JKS 1998 eqs 12-13, PRD 58 063001), the docs-held known
 answers (docs/search_for_candidates.md:377-438), invariants of FStat, and
 hand-built F vectors with known MAX_ALG-3 triggers."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -139,44 +141,54 @@ def test_rogcvir_python_host_matches_oracle():
         np.testing.assert_allclose(S.rogcvir(lat, gam), O.rogcvir(lat, gam), rtol=0, atol=1e-16)
 
 
-# ---------------- docs known answers ----------------
-DOCS_M = np.array([5.99211e-06, 1.16998e-26, -2.346e-17, 2.39069e-15, -7.00372e-06, 7.45542e-12, 0, 0,
-                   0.0026634, 1.47568e-11, -27.1918, 0, 0.000487897, -4.3489e-11, -3.79525, -9.09952])
+# ---------------- docs known answers (tests/golden/docs_known_answers.json) ----------------
+with open(os.path.join(os.path.dirname(__file__), "golden", "docs_known_answers.json")) as _f:
+    KAT = json.load(_f)
+DOCS_M = np.array(KAT["grid_matrix"]["M"])
 
 
 def test_search_settings_docs_kat():
-    """docs/search_for_candidates.md:377-403 (band 17, overlap 0.0625,
-    dt 0.5, nod 6, fftpad 1)."""
-    sett = S.SearchSettings(fpo=S.fpo(17, 0.0625, 0.5), dt=0.5, nod=6, fftpad=1)
+    """search_settings of the documented example (band 17, overlap 0.0625,
+    dt 0.5, nod 6, fftpad 1), docs/search_for_candidates.md:377-403."""
+    k = KAT["search_settings"]
+    i, o = k["input"], k["output"]
+    sett = S.SearchSettings(fpo=S.fpo(i["band"], i["overlap"], i["dt"]), dt=i["dt"], nod=i["nod"],
+                            fftpad=i["fftpad"])
     S.search_settings(sett)
-    assert sett.fpo == 25.9375
-    assert (sett.N, sett.nfft, sett.nfftf, sett.nmin, sett.nmax, sett.dd) == (1033969, 1048576, 1048576, 4096,
-                                                                               520192, 5)
-    assert sett.Ninterp == 2097152 and sett.interpftpad == 2
-    assert sett.Smax == pytest.approx(6.70414e-10, rel=1e-5)
-    assert sett.Smin == 0.0
-    assert sett.oms == pytest.approx(81.4851, rel=1e-6)
-    assert sett.omr == pytest.approx(3.64606e-05, rel=1e-5)
-    assert sett.nfftf // 16 == 65536  # bufsize in allsky mode (jobcore.c:59-61)
-    assert math.sin(S.C_EPSMA) == pytest.approx(0.397777, abs=1e-6)
-    assert math.cos(S.C_EPSMA) == pytest.approx(0.917482, abs=1e-6)
+    assert sett.fpo == o["fpo"]
+    for name in ("N", "nfft", "nfftf", "nmin", "nmax", "dd", "Ninterp", "interpftpad"):
+        assert getattr(sett, name) == o[name], name
+    assert sett.Smax == pytest.approx(o["Smax"], rel=1e-5)
+    assert sett.Smin == o["Smin"]
+    assert sett.oms == pytest.approx(o["oms"], rel=1e-6)
+    assert sett.omr == pytest.approx(o["omr"], rel=1e-5)
+    assert sett.nfftf // 16 == o["bufsize_allsky"]  # jobcore.c:59-61
+    assert math.sin(S.C_EPSMA) == pytest.approx(o["sepsm"], abs=1e-6)
+    assert math.cos(S.C_EPSMA) == pytest.approx(o["cepsm"], abs=1e-6)
 
 
 def test_trigger_record_docs_kat():
-    """Record (m=-8, n=0, s=-136) of docs/search_for_candidates.md:430-438:
-    ra 0.264512, dec 0.388286, fdot -8.4801e-10, computed by the oracle's
-    job_core scalar path (lin2ast, jobcore.c:242-262, 437-447)."""
+    """Record (m=-8, n=0, s=-136) of docs/search_for_candidates.md:430-438
+    computed by the oracle's job_core scalar path (lin2ast, jobcore.c:242-262,
+    437-447) from the documented grid matrix (6 printed digits, hence the
+    tolerance)."""
+    k = KAT["trigger_record"]
+    i, o = k["input"], k["output"]
     seg = synth.make_segment(nod=6, dt=0.5, band=17, overlap=0.0625, N=6000, det_names=("H1", "L1"),
                              seed=3, M=DOCS_M)
     sett = seg.sett
-    opts = TS.JobOpts(fr0=1000, fr1=3000, thr=1e9, sst=-136, spndr1=-136, sstep=1, bufsize=8192)
+    opts = TS.JobOpts(fr0=1000, fr1=3000, thr=1e9, sst=i["s"], spndr1=i["s"], sstep=1, bufsize=8192)
     orc = O.Oracle(sett, seg.dets)
-    orc.sky_stage(1, -8.0, 0.0, opts)
+    orc.sky_stage(i["pm"], float(i["m"]), float(i["n"]), opts)
     rec = orc.spindown(0, opts)["rec"]
-    assert rec[0] == -8 and rec[1] == 0 and rec[2] == -136
-    assert rec[3] == pytest.approx(0.264512, abs=3e-6)
-    assert rec[4] == pytest.approx(0.388286, abs=3e-6)
-    assert rec[5] == pytest.approx(-8.4801e-10, rel=2e-5)
+    assert rec[0] == i["m"] and rec[1] == i["n"] and rec[2] == i["s"]
+    assert rec[3] == pytest.approx(o["ra"], abs=3e-6)
+    assert rec[4] == pytest.approx(o["dec"], abs=3e-6)
+    assert rec[5] == pytest.approx(o["fdot"], rel=2e-5)
+
+
+def test_two_day_grid_matrix_is_the_documented_one():
+    np.testing.assert_array_equal(synth.M_2DAY_DT2, np.array(KAT["grid_2day_dt2"]["M"]))
 
 
 def test_lin2ast_is_on_the_sphere_and_inverts_ast2lin():
