@@ -127,7 +127,7 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
         step(k)
     ctx.synchronize()
     if profile:
-        ctx.profile(True)
+        ctx.profile(True, spindown_only=True)
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -254,9 +254,10 @@ int tdf_plan_info(tdf_ctx *ctx, int *n1, int *e1, int *e2, int *fused, int *batc
 int tdf_set_plan(tdf_ctx *ctx, int n1, int e1, int e2, int fused, int batch);
 
 /* ---- timing ---- */
-/* When enabled, every kernel launch of the spindown loop is bracketed
- * by HIP events on the context's stream; tdf_kernel_times returns, per
- * kernel slot, the summed milliseconds and launch count. */
+/* on = 1: every kernel launch (per-sky stage and spindown loop) is
+ * bracketed by HIP events on the stream it runs on; on = 2: only the
+ * spindown-loop kernels; 0: off.  tdf_kernel_times returns, per kernel
+ * slot, the summed milliseconds and launch count. */
 int tdf_profile_enable(tdf_ctx *ctx, int on);
 int tdf_kernel_times(tdf_ctx *ctx, int slot, double *ms, long long *launches,
                      const char **name);

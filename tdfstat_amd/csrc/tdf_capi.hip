@@ -115,10 +115,11 @@ struct tdf_ctx {
     hipEvent_t evS[2] = {nullptr, nullptr}, evA[2] = {nullptr, nullptr};
     int *cnt_hb[2] = {nullptr, nullptr};
     TrigEntry *trig_hb[2] = {nullptr, nullptr};
+    TrigEntry *trig_ds[2] = {nullptr, nullptr};  // device trigger lists of the two slots
     double *scal_hb[2] = {nullptr, nullptr};
     int cnt_hb_cap = 0;
-    // profiling
-    bool prof = false;
+    // profiling: bit 0 spindown kernels, bit 1 per-sky stage kernels
+    int prof = 0;
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_slot;
     size_t ev_used = 0;
@@ -154,9 +155,12 @@ struct Prof {
     int slot;
     size_t idx;
     hipStream_t st;
-    Prof(tdf_ctx *c_, int s, hipStream_t st_ = nullptr) : c(c_), slot(s), idx(0), st(st_ ? st_ : c_->stream)
+    bool on;
+    Prof(tdf_ctx *c_, int s, hipStream_t st_ = nullptr)
+        : c(c_), slot(s), idx(0), st(st_ ? st_ : c_->stream),
+          on((c_->prof & (s >= KS_SPIN1 ? 1 : 2)) != 0)
     {
-        if (!c->prof) return;
+        if (!on) return;
         if (c->ev_used + 2 > c->ev.size()) {
             hipEvent_t a, b;
             (void)hipEventCreate(&a);
@@ -172,7 +176,7 @@ struct Prof {
     }
     ~Prof()
     {
-        if (!c->prof) return;
+        if (!on) return;
         (void)hipEventRecord(c->ev[idx + 1], st);
         c->ev_used += 2;
     }
@@ -328,7 +332,7 @@ int check_opts(tdf_ctx *c, const tdf_job_opts *o)
 // triggers for each batch of up to c->batch templates.  No host sync.
 template <typename T>
 int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, const SkyScal &sk,
-                          bool use_cand, int *cnt_dev, int ncnt_tmpl)
+                          bool use_cand, int *cnt_dev, int ncnt_tmpl, int nzero = 0)
 {
     hipStream_t st = c->stream;
     const FftPlan &p = c->p_spin;
@@ -356,6 +360,8 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
         sa.cand = c->cand_d;
         sa.cand_cnt = cand_cnt;
         sa.cand_cap = (int)c->cand_cap;
+        sa.zero_cnt = cnt_dev;
+        sa.nzero = b0 == t0 ? nzero : 0;
         {
             Prof pr(c, KS_SPIN1);
             HIPCHK(launch_spin_pass1<T>(p, sa, st));
@@ -804,7 +810,7 @@ int tdf_spindown(tdf_ctx *c, int iss_begin, int iss_count, const tdf_job_opts *o
 
 namespace {
 
-constexpr int BATCH_TRIG_PREFIX = 1 << 16;  // triggers read back per sky point without a sync
+constexpr int BATCH_TRIG_PREFIX = 4096;  // triggers read back per sky point without a sync
 
 // Second per-sky slot, sky stream, events and pinned readback buffers of
 // the multi-sky pipeline; allocated on first use.
@@ -816,7 +822,9 @@ int ensure_batch(tdf_ctx *c, int ncnt)
         c->xm_s[0] = c->xm;
         c->tmp1_s[0] = c->tmp1;
         c->scal_s[0] = c->sky_scal;
+        c->trig_ds[0] = c->trig_d;
         int rc;
+        if ((rc = dmalloc(c, (void **)&c->trig_ds[1], sizeof(TrigEntry) * c->trig_cap))) return rc;
         if ((rc = dmalloc(c, &c->xm_s[1], c->csz * 2 * g.nifo * (size_t)g.nfftf))) return rc;
         if ((rc = dmalloc(c, (void **)&c->tmp1_s[1], sizeof(double) * g.nifo * (size_t)g.nfftf))) return rc;
         if ((rc = dmalloc(c, (void **)&c->scal_s[1], sizeof(double) * 4))) return rc;
@@ -849,6 +857,7 @@ void use_slot(tdf_ctx *c, int slot)
     c->xm = c->xm_s[slot];
     c->tmp1 = c->tmp1_s[slot];
     c->sky_scal = c->scal_s[slot];
+    c->trig_d = c->trig_ds[slot];
 }
 
 // Records of one spindown call from its trigger list (ascending bin per
@@ -947,8 +956,8 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
         if (iss > 0) {
             const SkyScal sk{S[j].smin, o->sstep, mm[j], nn[j], S[j].het0, M[0], M[4], M[5], M[9], M[13], 0};
             const int nc = 1 + iss + (iss + c->batch - 1) / c->batch;
-            HIPCHK(hipMemsetAsync(c->cnt_d, 0, sizeof(int) * nc, c->stream));
-            int r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_d, iss);
+            // counters cleared by the first pass-1 launch (no memset call)
+            int r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_d, iss, nc);
             if (r) return r;
             HIPCHK(hipMemcpyAsync(c->cnt_hb[slot], c->cnt_d, sizeof(int) * nc, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipMemcpyAsync(c->trig_hb[slot], c->trig_d, sizeof(TrigEntry) * prefix, hipMemcpyDeviceToHost,
@@ -972,14 +981,23 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
             for (int b = 0; b < (iss + c->batch - 1) / c->batch; b++)
                 if (cnt[1 + iss + b] > c->cand_cap) again = true;
         const long long ntot = cnt[0];
-        if (ntot > prefix) again = true;
+        if (ntot > c->trig_cap) again = true;
         if (again || overflow) {
             if (!overflow) redo.push_back(j);
             return TDF_OK;
         }
         nrec_per[j] = iss;
-        return fill_records(c, S[j], mm[j], nn[j], o, iss, c->trig_hb[slot], ntot, recs + roff[j], pairs,
-                            pair_cap, &np);
+        if (ntot <= prefix)
+            return fill_records(c, S[j], mm[j], nn[j], o, iss, c->trig_hb[slot], ntot, recs + roff[j], pairs,
+                                pair_cap, &np);
+        // the rest of a long list: this slot's device list is not written again
+        // before the next use of the slot, which is enqueued after this call
+        std::vector<TrigEntry> all((size_t)ntot);
+        std::copy(c->trig_hb[slot], c->trig_hb[slot] + prefix, all.begin());
+        HIPCHK(hipMemcpy(all.data() + prefix, c->trig_ds[slot] + prefix, sizeof(TrigEntry) * (ntot - prefix),
+                         hipMemcpyDeviceToHost));
+        return fill_records(c, S[j], mm[j], nn[j], o, iss, all.data(), ntot, recs + roff[j], pairs, pair_cap,
+                            &np);
     };
 
     const int n = (int)vis.size();
@@ -1222,8 +1240,8 @@ int tdf_autotune(tdf_ctx *c)
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const bool prof = c->prof;
-    c->prof = false;
+    const int prof = c->prof;
+    c->prof = 0;
     FftPlan best = c->p_spin;
     int best_b = c->batch;
     float best_t = 1e30f;
@@ -1461,7 +1479,7 @@ int tdf_sincos_probe(const double *ph, int n, int mode, float *s, float *c)
 int tdf_profile_enable(tdf_ctx *c, int on)
 {
     if (!c) return TDF_ERR_ARG;
-    c->prof = on != 0;
+    c->prof = on == 1 ? 3 : on == 2 ? 1 : 0;  // 1: every kernel, 2: spindown kernels only
     for (int k = 0; k < KS_COUNT; k++) {
         c->kms[k] = 0.;
         c->kcount[k] = 0;

@@ -148,6 +148,9 @@ struct SpinArgs {
     TrigEntry *cand;
     int *cand_cnt;
     int cand_cap;
+    // counters cleared by workgroup (0, 0) of pass 1 (nzero = 0: none)
+    int *zero_cnt;
+    int nzero;
 };
 
 struct FstatArgs {

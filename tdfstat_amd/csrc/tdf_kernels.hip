@@ -253,10 +253,14 @@ struct IO_SpinP1 {
     cx<T> *Z;              // [batch][2][n]
     const cx<T> *tw_lo, *tw_hi;
     int N, n, n1, nifo, slot0;
+    int *zero_cnt;  // cleared by workgroup (0, 0) before the kernels that count
+    int nzero;
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
         constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
+        if (nzero > 0 && blockIdx.x == 0 && blockIdx.y == 0)
+            for (int k = threadIdx.x; k < nzero; k += NT) zero_cnt[k] = 0;
         const TmplDesc td = tmpl_scalars(sk, sk.iss0 + slot0 + blockIdx.y);
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         const int nd = NIFO > 0 ? NIFO : nifo;
@@ -937,7 +941,7 @@ static hipError_t launch_spin_pass1_n(const FftPlan &p, const SpinArgs &s, hipSt
 {
     IO_SpinP1<T, NIFO, HALF> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z,
                           (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D], (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D],
-                          s.N, p.n, p.n1, s.nifo, s.slot0};
+                          s.N, p.n, p.n1, s.nifo, s.slot0, s.zero_cnt, s.nzero};
     const int tc1 = p.e1 / p.n2 / 2;
     return launch_pass<T, -1>(p.n2, p.e1, io, dim3(p.n1 / tc1, s.nslots),
                               (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2], st);

@@ -271,8 +271,9 @@ class Context:
         L.check(self.lib.tdf_set_plan(self.h, n1, e1, e2, int(fused), batch), "tdf_set_plan")
         self.batch = batch
 
-    def profile(self, on=True):
-        L.check(self.lib.tdf_profile_enable(self.h, int(on)), "tdf_profile_enable")
+    def profile(self, on=True, spindown_only=False):
+        mode = (2 if spindown_only else 1) if on else 0
+        L.check(self.lib.tdf_profile_enable(self.h, mode), "tdf_profile_enable")
 
     def kernel_times(self):
         out = {}

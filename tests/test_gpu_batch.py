@@ -20,7 +20,7 @@ POINTS = [(1, 0.0, 0.0), (2, 1.0, -2.0), (1, 400.0, 400.0), (1, -3.0, 2.0), (2, 
 
 
 @pytest.mark.parametrize("prec", [L.TDF_PREC_FP32, L.TDF_PREC_FP64])
-@pytest.mark.parametrize("thr", [20.0, 2.0])
+@pytest.mark.parametrize("thr", [20.0, 5.0, 2.0])
 def test_batch_equals_per_point(prec, thr):
     _flush(f"batch prec={prec} thr={thr}")
     rng = np.random.default_rng(31)
