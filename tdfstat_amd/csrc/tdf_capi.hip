@@ -16,6 +16,9 @@
 
 using namespace tdf;
 
+// sky points per sky-stage launch sequence in tdf_job_core_batch (at most)
+constexpr int GMAX_SKY = 16;
+
 // The per-sky and per-template scalars (het0, het1, spnd, sgnl0, f_) must
 // round exactly like the reference's separate multiply/add operations.
 #pragma clang fp contract(off)
@@ -107,15 +110,19 @@ struct tdf_ctx {
     std::vector<double> lines_h;
     int last_batch_n = 0, last_batch_iss0 = 0, call_iss0 = 0;
     int last_fr0 = 0, last_fr1 = -1, last_nblk = 0, last_nav = 1;
-    // multi-sky pipeline (tdf_job_core_batch): second per-sky slot, sky
-    // stream, events, double-buffered pinned readback
+    // per-sky constants: [0] single-point path, [1 + slot*GMAX + k] groups
+    SkySite *site_d = nullptr, *site_h = nullptr;
+    // multi-sky pipeline (tdf_job_core_batch): sky stage of a group of gmax
+    // points into one of two slots, sky stream, events, pinned read-back
+    int gmax = 0;
     hipStream_t sstream = nullptr;
-    void *xm_s[2] = {nullptr, nullptr};
-    double *tmp1_s[2] = {nullptr, nullptr}, *scal_s[2] = {nullptr, nullptr};
+    void *xab_g = nullptr, *Zsky_g = nullptr, *XT_g = nullptr, *Ymv_g = nullptr;
+    double *shftf_g = nullptr, *partials_g = nullptr;
+    void *xm_g[2] = {nullptr, nullptr};
+    double *tmp1_g[2] = {nullptr, nullptr}, *scal_g[2] = {nullptr, nullptr};
     hipEvent_t evS[2] = {nullptr, nullptr}, evA[2] = {nullptr, nullptr};
-    int *cnt_hb[2] = {nullptr, nullptr};
-    TrigEntry *trig_hb[2] = {nullptr, nullptr};
-    TrigEntry *trig_ds[2] = {nullptr, nullptr};  // device trigger lists of the two slots
+    int *cnt_hb[2] = {nullptr, nullptr}, *cnt_g[2] = {nullptr, nullptr};  // host / device, gmax x cnt_hb_cap
+    TrigEntry *trig_hb[2] = {nullptr, nullptr}, *trig_g[2] = {nullptr, nullptr};  // gmax x prefix
     double *scal_hb[2] = {nullptr, nullptr};
     int cnt_hb_cap = 0;
     // profiling: bit 0 spindown kernels, bit 1 per-sky stage kernels
@@ -262,53 +269,79 @@ int make_plan(tdf_ctx *c, int n, FftPlan *p, int n1_req = 0, int e1 = 0, int e2 
 
 SpinLayout spin_layout(const tdf_ctx *c) { return SpinLayout{c->p_spin.n1, c->p_spin.l1, c->p_spin.n2}; }
 
+// Buffers of one sky-stage launch sequence over G sky points.
+struct SkyBufs {
+    void *xab, *Zsky, *XT, *Ymv, *xm;
+    double *shftf, *partials, *tmp1, *scal;
+    const SkySite *site;  // device, G entries
+};
+
+SkyBufs single_bufs(tdf_ctx *c)
+{
+    return SkyBufs{c->xab, c->Zsky, c->XT, c->Ymv, c->xm, c->shftf, c->partials, c->tmp1, c->sky_scal, c->site_d};
+}
+
+// Per-sky stage (jobcore.c:242-391) of G sky points with one launch per step.
 template <typename T>
-int run_sky_stage(tdf_ctx *c, const SkyPrepArgs &a, hipStream_t st)
+int run_sky_stage(tdf_ctx *c, const SkyBufs &b, int G, hipStream_t st)
 {
     const tdf_geom &g = c->g;
     const int nifo = g.nifo;
+    SkyPrepArgs a{};
+    a.lay = spin_layout(c);
+    a.xDat = c->xDat;
+    a.DetSSB = c->DetSSB;
+    a.det = c->detp;
+    a.site = b.site;
+    a.shftf = b.shftf;
+    a.tmp1 = b.tmp1;
+    a.partials = b.partials;
+    a.N = g.N;
+    a.nfft = g.nfft;
+    a.omr = g.omr;
+    a.oms = g.oms;
     {
         Prof pr(c, KS_SKY_PREP, st);
-        HIPCHK(launch_sky_prep<T>(a, c->xab, c->prep_nblk, nifo, st));
+        HIPCHK(launch_sky_prep<T>(a, b.xab, c->prep_nblk, nifo, G, st));
     }
     {
         Prof pr(c, KS_SKY_REDUCE, st);
-        HIPCHK(launch_sky_reduce(c->partials, c->prep_nblk, nifo, c->sig2_d, c->sky_scal, st));
+        HIPCHK(launch_sky_reduce(b.partials, c->prep_nblk, nifo, c->sig2_d, b.scal, G, st));
     }
     // forward FFT of length nfft, result column-major in XT
     {
         Prof pr(c, KS_SKY_FFT, st);
-        HIPCHK(launch_fft_natural_to_T<T>(c->p_fwd, c->xab, c->Zsky, c->XT, nifo, -1, 0, 0, 1.0, st));
+        HIPCHK(launch_fft_natural_to_T<T>(c->p_fwd, b.xab, b.Zsky, b.XT, nifo * G, -1, 0, 0, 1.0, st));
     }
     // natural order + spectrum move into Ymv (jobcore.c:340-350)
     const int nyqst = g.nfft / 2 + 1;
     {
         Prof pr(c, KS_SKY_TRANSPOSE, st);
-        HIPCHK(launch_transpose<T>(c->p_fwd, c->XT, c->Ymv, 2 * nifo, (size_t)g.nfft, (size_t)g.Ninterp, nyqst,
+        HIPCHK(launch_transpose<T>(c->p_fwd, b.XT, b.Ymv, 2 * nifo * G, (size_t)g.nfft, (size_t)g.Ninterp, nyqst,
                                    g.Ninterp - g.nfft, g.nfft, st));
     }
     // inverse FFT of length Ninterp, scaled by interpftpad/Ninterp
     const double ft = (double)g.interpftpad / g.Ninterp;
     {
         Prof pr(c, KS_SKY_IFFT, st);
-        HIPCHK(launch_fft_natural_to_T<T>(c->p_inv, c->Ymv, c->Zsky, c->XT, nifo, +1, nyqst,
+        HIPCHK(launch_fft_natural_to_T<T>(c->p_inv, b.Ymv, b.Zsky, b.XT, nifo * G, +1, nyqst,
                                           nyqst + g.Ninterp - g.nfft, ft, st));
     }
     const int nnodes = g.interpftpad * g.N;
     {
         Prof pr(c, KS_SKY_TRANSPOSE2, st);
-        HIPCHK(launch_transpose<T>(c->p_inv, c->XT, c->Ymv, 2 * nifo, (size_t)g.Ninterp, (size_t)g.Ninterp,
+        HIPCHK(launch_transpose<T>(c->p_inv, b.XT, b.Ymv, 2 * nifo * G, (size_t)g.Ninterp, (size_t)g.Ninterp,
                                    g.Ninterp, 0, std::min(g.Ninterp, nnodes + 1), st));
     }
     {
         Prof pr(c, KS_SPLINE, st);
-        HIPCHK(launch_spline<T>(c->Ymv, c->Zsky, nnodes, (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, c->taps,
-                                st));
+        HIPCHK(launch_spline<T>(b.Ymv, b.Zsky, nnodes, (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo * G,
+                                c->taps, st));
     }
     {
         Prof pr(c, KS_SPLINT, st);
-        HIPCHK(launch_splint<T>(c->Ymv, c->Zsky, c->shftf, c->sig2_d, c->xm, g.N, nnodes, g.interpftpad,
-                                (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo, spin_layout(c),
+        HIPCHK(launch_splint<T>(b.Ymv, b.Zsky, b.shftf, c->sig2_d, b.xm, g.N, nnodes, g.interpftpad,
+                                (size_t)g.Ninterp, (size_t)g.Ninterp, 2 * nifo * G, nifo, spin_layout(c),
                                 (size_t)g.nfftf, st));
     }
     return TDF_OK;
@@ -332,8 +365,13 @@ int check_opts(tdf_ctx *c, const tdf_job_opts *o)
 // triggers for each batch of up to c->batch templates.  No host sync.
 template <typename T>
 int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, const SkyScal &sk,
-                          bool use_cand, int *cnt_dev, int ncnt_tmpl, int nzero = 0)
+                          bool use_cand, int *cnt_dev, int ncnt_tmpl, int nzero = 0, TrigEntry *trig_out = nullptr,
+                          long long trig_cap = 0)
 {
+    if (!trig_out) {
+        trig_out = c->trig_d;
+        trig_cap = c->trig_cap;
+    }
     hipStream_t st = c->stream;
     const FftPlan &p = c->p_spin;
     const tdf_geom &g = c->g;
@@ -390,15 +428,15 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
         ta.nav = o->nav > 0 ? o->nav : 1;
         ta.nblk_cap = c->nblk_cap;
         ta.numlines = o->numlines;
-        ta.cap = (int)std::min<long long>(c->trig_cap, 0x7fffffff);
+        ta.cap = (int)std::min<long long>(trig_cap, 0x7fffffff);
         ta.lines = c->lines_d;
         ta.sk = sk;
         if (use_cand) {
             Prof pr(c, KS_TRIG);
-            HIPCHK(launch_candidates<T>(c->FT, ta, c->cand_d, cand_cnt, (int)c->cand_cap, c->trig_d, cnt_dev, st));
+            HIPCHK(launch_candidates<T>(c->FT, ta, c->cand_d, cand_cnt, (int)c->cand_cap, trig_out, cnt_dev, st));
         } else {
             Prof pr(c, KS_TRIG);
-            HIPCHK(launch_triggers<T>(c->FT, c->mu, ta, nslots, c->trig_d, cnt_dev, st));
+            HIPCHK(launch_triggers<T>(c->FT, c->mu, ta, nslots, trig_out, cnt_dev, st));
         }
         c->last_batch_n = nslots;
         c->last_batch_iss0 = c->call_iss0 + b0;
@@ -415,17 +453,6 @@ int ensure_counters(tdf_ctx *c, int n)
     const int cap = std::max(need, 2 * c->cnt_cap);
     if (c->cnt_d) (void)hipFree(c->cnt_d);
     if (c->cnt_h) (void)hipHostFree(c->cnt_h);
-    for (int k = 0; k < 2; k++) {
-        if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
-        if (c->trig_hb[k]) (void)hipHostFree(c->trig_hb[k]);
-        if (c->scal_hb[k]) (void)hipHostFree(c->scal_hb[k]);
-        if (c->evS[k]) (void)hipEventDestroy(c->evS[k]);
-        if (c->evA[k]) (void)hipEventDestroy(c->evA[k]);
-    }
-    if (c->sstream) {
-        (void)hipStreamSynchronize(c->sstream);
-        (void)hipStreamDestroy(c->sstream);
-    }
     c->cnt_d = nullptr;
     c->cnt_h = nullptr;
     c->cnt_cap = 0;
@@ -438,11 +465,12 @@ int ensure_counters(tdf_ctx *c, int n)
 // Host part of the sky stage (jobcore.c:242-262, 399-411): visibility,
 // lin2ast, het0, the spindown range and the per-sky kernel arguments.  aa
 // and bb are left to the device.
-void sky_setup(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o, tdf_sky *sp, SkyPrepArgs *ap)
+void sky_setup(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o, tdf_sky *sp, SkySite *ap)
 {
     tdf_sky &s = *sp;
-    SkyPrepArgs &a = *ap;
+    SkySite &a = *ap;
     s = tdf_sky{};
+    a = SkySite{};
     const tdf_geom &g = c->g;
     const double *M = g.M;
     // jobcore.c:242-248
@@ -463,17 +491,6 @@ void sky_setup(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o, td
     s.dec = asin(sindel);
     s.ra = fmod(atan2(sinal, cosal) + 2. * M_PI, 2. * M_PI);
     s.het0 = fmod(nn * M[8] + mm * M[12], M[0]);
-    a.lay = spin_layout(c);
-    a.xDat = c->xDat;
-    a.DetSSB = c->DetSSB;
-    a.det = c->detp;
-    a.shftf = c->shftf;
-    a.tmp1 = c->tmp1;  // current slot
-    a.partials = c->partials;
-    a.N = g.N;
-    a.nfft = g.nfft;
-    a.omr = g.omr;
-    a.oms = g.oms;
     a.het0 = s.het0;
     a.ns0 = cosal * cosdel;
     a.ns1 = sinal * cosdel;
@@ -534,10 +551,21 @@ int tdf_ctx_destroy(tdf_ctx *c)
 {
     if (!c) return TDF_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->sstream) (void)hipStreamSynchronize(c->sstream);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->cnt_d) (void)hipFree(c->cnt_d);
     if (c->trig_h) (void)hipHostFree(c->trig_h);
     if (c->cnt_h) (void)hipHostFree(c->cnt_h);
+    if (c->site_h) (void)hipHostFree(c->site_h);
+    for (int k = 0; k < 2; k++) {
+        if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
+        if (c->cnt_g[k]) (void)hipFree(c->cnt_g[k]);
+        if (c->trig_hb[k]) (void)hipHostFree(c->trig_hb[k]);
+        if (c->scal_hb[k]) (void)hipHostFree(c->scal_hb[k]);
+        if (c->evS[k]) (void)hipEventDestroy(c->evS[k]);
+        if (c->evA[k]) (void)hipEventDestroy(c->evA[k]);
+    }
+    if (c->sstream) (void)hipStreamDestroy(c->sstream);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -602,6 +630,8 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
     c->prep_nblk = (g->nfft + 255) / 256;
     TRY(dmalloc(c, (void **)&c->partials, sizeof(double) * 2 * nifo * (size_t)c->prep_nblk));
     TRY(dmalloc(c, (void **)&c->sky_scal, sizeof(double) * 4));
+    TRY(dmalloc(c, (void **)&c->site_d, sizeof(SkySite) * (1 + 2 * GMAX_SKY)));
+    if (hipHostMalloc((void **)&c->site_h, sizeof(SkySite) * (1 + 2 * GMAX_SKY), 0) != hipSuccess) TRY(TDF_ERR_NOMEM);
     {
         // spline taps: 1/(2 sqrt 3) (-r)^k, k <= K, then (-r)^i, i < BK
         const double r = 2. - sqrt(3.);
@@ -669,14 +699,16 @@ int tdf_sky_stage(tdf_ctx *c, int pm, float mm, float nn, const tdf_job_opts *o,
         if (!c->det_set[d]) return TDF_ERR_STATE;
     c->sky_valid = false;
     tdf_sky s{};
-    SkyPrepArgs a{};
-    sky_setup(c, pm, mm, nn, o, &s, &a);
+    sky_setup(c, pm, mm, nn, o, &s, &c->site_h[0]);
     if (!s.visible) {
         *sky = s;
         c->sky = s;
         return TDF_OK;
     }
-    rc = c->prec == TDF_PREC_FP64 ? run_sky_stage<double>(c, a, c->stream) : run_sky_stage<float>(c, a, c->stream);
+    HIPCHK(hipMemcpyAsync(c->site_d, c->site_h, sizeof(SkySite), hipMemcpyHostToDevice, c->stream));
+    const SkyBufs b = single_bufs(c);
+    rc = c->prec == TDF_PREC_FP64 ? run_sky_stage<double>(c, b, 1, c->stream)
+                                  : run_sky_stage<float>(c, b, 1, c->stream);
     if (rc) return rc;
     double scal[2];
     HIPCHK(hipMemcpyAsync(scal, c->sky_scal, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
@@ -783,6 +815,11 @@ int tdf_spindown(tdf_ctx *c, int iss_begin, int iss_count, const tdf_job_opts *o
             }
             for (long long t = 0; t < ntot; t++) {
                 const TrigEntry &e = c->trig_h[t];
+                if (e.slot < 0 || e.slot >= iss_count) {
+                    fprintf(stderr, "[tdfstat_hip] trigger %lld of %lld names template %d of %d\n", t, ntot, e.slot,
+                            iss_count);
+                    return TDF_ERR_HIP;
+                }
                 per[e.slot].push_back({e.ic, {e.f, e.F}});
             }
         }
@@ -812,52 +849,74 @@ namespace {
 
 constexpr int BATCH_TRIG_PREFIX = 4096;  // triggers read back per sky point without a sync
 
-// Second per-sky slot, sky stream, events and pinned readback buffers of
-// the multi-sky pipeline; allocated on first use.
+// Group sky-stage buffers (gmax points), two slots of per-point spindown
+// inputs, the sky stream, events and pinned read-back; allocated on first
+// use.  gmax: as many points as fit 512 MB of sky-stage buffers, <= 16.
 int ensure_batch(tdf_ctx *c, int ncnt)
 {
     const tdf_geom &g = c->g;
+    const size_t cs = c->csz;
+    const int nifo = g.nifo;
     if (!c->sstream) {
+        const size_t per = cs * 2 * nifo * ((size_t)g.nfft + 3 * (size_t)g.Ninterp) + 8 * nifo * (size_t)g.N;
+        c->gmax = (int)std::max<size_t>(1, std::min<size_t>(GMAX_SKY, (512ull << 20) / per));
+        const size_t G = c->gmax;
         HIPCHK(hipStreamCreateWithFlags(&c->sstream, hipStreamNonBlocking));
-        c->xm_s[0] = c->xm;
-        c->tmp1_s[0] = c->tmp1;
-        c->scal_s[0] = c->sky_scal;
-        c->trig_ds[0] = c->trig_d;
         int rc;
-        if ((rc = dmalloc(c, (void **)&c->trig_ds[1], sizeof(TrigEntry) * c->trig_cap))) return rc;
-        if ((rc = dmalloc(c, &c->xm_s[1], c->csz * 2 * g.nifo * (size_t)g.nfftf))) return rc;
-        if ((rc = dmalloc(c, (void **)&c->tmp1_s[1], sizeof(double) * g.nifo * (size_t)g.nfftf))) return rc;
-        if ((rc = dmalloc(c, (void **)&c->scal_s[1], sizeof(double) * 4))) return rc;
-        HIPCHK(hipMemset(c->xm_s[1], 0, c->csz * 2 * g.nifo * (size_t)g.nfftf));
-        HIPCHK(hipMemset(c->tmp1_s[1], 0, sizeof(double) * g.nifo * (size_t)g.nfftf));
+        if ((rc = dmalloc(c, &c->xab_g, G * cs * 2 * nifo * (size_t)g.nfft))) return rc;
+        if ((rc = dmalloc(c, &c->Zsky_g, G * cs * 2 * nifo * (size_t)g.Ninterp))) return rc;
+        if ((rc = dmalloc(c, &c->XT_g, G * cs * 2 * nifo * (size_t)g.Ninterp))) return rc;
+        if ((rc = dmalloc(c, &c->Ymv_g, G * cs * 2 * nifo * (size_t)g.Ninterp))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->shftf_g, G * sizeof(double) * nifo * (size_t)g.N))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->partials_g, G * sizeof(double) * 2 * nifo * (size_t)c->prep_nblk)))
+            return rc;
         for (int k = 0; k < 2; k++) {
+            if ((rc = dmalloc(c, &c->xm_g[k], G * cs * 2 * nifo * (size_t)g.nfftf))) return rc;
+            if ((rc = dmalloc(c, (void **)&c->tmp1_g[k], G * sizeof(double) * nifo * (size_t)g.nfftf))) return rc;
+            if ((rc = dmalloc(c, (void **)&c->scal_g[k], G * sizeof(double) * 4))) return rc;
+            // padding beyond N is read by the spindown kernels: keep it zero
+            HIPCHK(hipMemset(c->xm_g[k], 0, G * cs * 2 * nifo * (size_t)g.nfftf));
+            HIPCHK(hipMemset(c->tmp1_g[k], 0, G * sizeof(double) * nifo * (size_t)g.nfftf));
             HIPCHK(hipEventCreateWithFlags(&c->evS[k], hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&c->evA[k], hipEventDisableTiming));
             // recorded once so that the first waits on them are satisfied
             HIPCHK(hipEventRecord(c->evS[k], c->sstream));
             HIPCHK(hipEventRecord(c->evA[k], c->stream));
-            if (hipHostMalloc((void **)&c->trig_hb[k], sizeof(TrigEntry) * BATCH_TRIG_PREFIX, 0) != hipSuccess ||
-                hipHostMalloc((void **)&c->scal_hb[k], sizeof(double) * 4, 0) != hipSuccess)
+            if ((rc = dmalloc(c, (void **)&c->trig_g[k], G * sizeof(TrigEntry) * BATCH_TRIG_PREFIX))) return rc;
+            if (hipHostMalloc((void **)&c->trig_hb[k], G * sizeof(TrigEntry) * BATCH_TRIG_PREFIX, 0) != hipSuccess ||
+                hipHostMalloc((void **)&c->scal_hb[k], G * sizeof(double) * 4, 0) != hipSuccess)
                 return TDF_ERR_NOMEM;
         }
     }
     if (ncnt > c->cnt_hb_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
         for (int k = 0; k < 2; k++) {
             if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
+            if (c->cnt_g[k]) (void)hipFree(c->cnt_g[k]);
             c->cnt_hb[k] = nullptr;
-            if (hipHostMalloc((void **)&c->cnt_hb[k], sizeof(int) * ncnt, 0) != hipSuccess) return TDF_ERR_NOMEM;
+            c->cnt_g[k] = nullptr;
+            if (hipHostMalloc((void **)&c->cnt_hb[k], sizeof(int) * ncnt * (size_t)c->gmax, 0) != hipSuccess ||
+                hipMalloc((void **)&c->cnt_g[k], sizeof(int) * ncnt * (size_t)c->gmax) != hipSuccess)
+                return TDF_ERR_NOMEM;
         }
         c->cnt_hb_cap = ncnt;
     }
     return TDF_OK;
 }
 
-void use_slot(tdf_ctx *c, int slot)
+SkyBufs group_bufs(tdf_ctx *c, int slot)
 {
-    c->xm = c->xm_s[slot];
-    c->tmp1 = c->tmp1_s[slot];
-    c->sky_scal = c->scal_s[slot];
-    c->trig_d = c->trig_ds[slot];
+    return SkyBufs{c->xab_g, c->Zsky_g, c->XT_g, c->Ymv_g, c->xm_g[slot], c->shftf_g, c->partials_g,
+                   c->tmp1_g[slot], c->scal_g[slot], c->site_d + 1 + slot * GMAX_SKY};
+}
+
+// point k of the group in slot: its spindown inputs become the current ones
+void use_point(tdf_ctx *c, int slot, int k)
+{
+    const tdf_geom &g = c->g;
+    c->xm = (char *)c->xm_g[slot] + (size_t)k * c->csz * 2 * g.nifo * g.nfftf;
+    c->tmp1 = c->tmp1_g[slot] + (size_t)k * g.nifo * g.nfftf;
+    c->sky_scal = c->scal_g[slot] + 4 * (size_t)k;
 }
 
 // Records of one spindown call from its trigger list (ascending bin per
@@ -869,7 +928,14 @@ int fill_records(tdf_ctx *c, const tdf_sky &sky, float mm, float nn, const tdf_j
     const double *M = c->g.M;
     const SkyScal sk{sky.smin, o->sstep, mm, nn, sky.het0, M[0], M[4], M[5], M[9], M[13], 0};
     std::vector<std::vector<std::pair<int, std::pair<float, float>>>> per(iss_count);
-    for (long long t = 0; t < ntot; t++) per[tr[t].slot].push_back({tr[t].ic, {tr[t].f, tr[t].F}});
+    for (long long t = 0; t < ntot; t++) {
+        if (tr[t].slot < 0 || tr[t].slot >= iss_count) {
+            fprintf(stderr, "[tdfstat_hip] trigger %lld of %lld names template %d of %d\n", t, ntot, tr[t].slot,
+                    iss_count);
+            return TDF_ERR_HIP;
+        }
+        per[tr[t].slot].push_back({tr[t].ic, {tr[t].f, tr[t].F}});
+    }
     for (int k = 0; k < iss_count; k++) {
         const TmplDesc td = tmpl_scalars(sk, k);
         tdf_record &r = recs[k];
@@ -901,13 +967,13 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
     const tdf_geom &g = c->g;
     const double *M = g.M;
     std::vector<tdf_sky> S(nsky);
-    std::vector<SkyPrepArgs> A(nsky);
+    std::vector<SkySite> site(nsky);
     std::vector<long long> roff(nsky, 0);
     std::vector<int> vis;
     long long rtot = 0;
     int maxiss = 1;
     for (int k = 0; k < nsky; k++) {
-        sky_setup(c, pm[k], mm[k], nn[k], o, &S[k], &A[k]);
+        sky_setup(c, pm[k], mm[k], nn[k], o, &S[k], &site[k]);
         nrec_per[k] = 0;
         roff[k] = rtot;
         if (S[k].visible) {
@@ -930,6 +996,11 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
                               hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    // groups of G visible points: at least three groups per call when there
+    // are enough points, so that sky stages overlap spindown loops
+    const int nv = (int)vis.size();
+    const int G = std::max(1, std::min(c->gmax, (nv + 2) / 3));
+    const int ngrp = (nv + G - 1) / G;
     const bool use_cand = !o->fstat_norm || (o->fr1 - o->fr0) / std::max(o->nav, 1) == 0;
     const int maxtrig = o->bufsize / 2;
     const long long prefix = std::min<long long>(c->trig_cap, BATCH_TRIG_PREFIX);
@@ -937,83 +1008,93 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
     bool overflow = false;
     long long np = 0;
     c->call_iss0 = 0;
+    void *xm0 = c->xm;
+    double *tmp10 = c->tmp1, *scal0 = c->sky_scal;
+    auto grp = [&](int gi, int k) { return vis[(size_t)gi * G + k]; };
+    auto gsize = [&](int gi) { return std::min(G, nv - gi * G); };
 
-    auto enqueue_sky = [&](int j, int slot) -> int {
-        use_slot(c, slot);
-        A[j].tmp1 = c->tmp1;
+    auto enqueue_sky = [&](int gi, int slot) -> int {
+        const int ng = gsize(gi);
+        SkySite *sh = c->site_h + 1 + slot * GMAX_SKY;
         HIPCHK(hipStreamWaitEvent(c->sstream, c->evA[slot], 0));
-        int r = run_sky_stage<T>(c, A[j], c->sstream);
+        for (int k = 0; k < ng; k++) sh[k] = site[grp(gi, k)];
+        const SkyBufs b = group_bufs(c, slot);
+        HIPCHK(hipMemcpyAsync((void *)b.site, sh, sizeof(SkySite) * ng, hipMemcpyHostToDevice, c->sstream));
+        int r = run_sky_stage<T>(c, b, ng, c->sstream);
         if (r) return r;
-        HIPCHK(hipMemcpyAsync(c->scal_hb[slot], c->scal_s[slot], sizeof(double) * 2, hipMemcpyDeviceToHost,
+        HIPCHK(hipMemcpyAsync(c->scal_hb[slot], c->scal_g[slot], sizeof(double) * 4 * ng, hipMemcpyDeviceToHost,
                               c->sstream));
         HIPCHK(hipEventRecord(c->evS[slot], c->sstream));
         return TDF_OK;
     };
-    auto enqueue_spin = [&](int j, int slot) -> int {
-        use_slot(c, slot);
+    auto enqueue_spin = [&](int gi, int slot) -> int {
         HIPCHK(hipStreamWaitEvent(c->stream, c->evS[slot], 0));
-        const int iss = S[j].iss_size;
-        if (iss > 0) {
+        const int ng = gsize(gi);
+        for (int k = 0; k < ng; k++) {
+            const int j = grp(gi, k);
+            const int iss = S[j].iss_size;
+            if (iss <= 0) continue;
+            use_point(c, slot, k);
             const SkyScal sk{S[j].smin, o->sstep, mm[j], nn[j], S[j].het0, M[0], M[4], M[5], M[9], M[13], 0};
             const int nc = 1 + iss + (iss + c->batch - 1) / c->batch;
+            // per-point counters and trigger list in the group's device area,
             // counters cleared by the first pass-1 launch (no memset call)
-            int r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_d, iss, nc);
+            int r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_g[slot] + (size_t)k * c->cnt_hb_cap,
+                                             iss, nc, c->trig_g[slot] + (size_t)k * BATCH_TRIG_PREFIX, prefix);
             if (r) return r;
-            HIPCHK(hipMemcpyAsync(c->cnt_hb[slot], c->cnt_d, sizeof(int) * nc, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipMemcpyAsync(c->trig_hb[slot], c->trig_d, sizeof(TrigEntry) * prefix, hipMemcpyDeviceToHost,
-                                  c->stream));
         }
+        // one read-back per group
+        HIPCHK(hipMemcpyAsync(c->cnt_hb[slot], c->cnt_g[slot], sizeof(int) * ng * (size_t)c->cnt_hb_cap,
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->trig_hb[slot], c->trig_g[slot], sizeof(TrigEntry) * ng * BATCH_TRIG_PREFIX,
+                              hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipEventRecord(c->evA[slot], c->stream));
         return TDF_OK;
     };
-    auto finish = [&](int j, int slot) -> int {
+    auto finish = [&](int gi, int slot) -> int {
         HIPCHK(hipEventSynchronize(c->evA[slot]));
-        S[j].aa = c->scal_hb[slot][0];
-        S[j].bb = c->scal_hb[slot][1];
-        skies[j] = S[j];
-        const int iss = S[j].iss_size;
-        if (iss <= 0) return TDF_OK;
-        const int *cnt = c->cnt_hb[slot];
-        for (int k = 0; k < iss; k++)
-            if (cnt[1 + k] > maxtrig) overflow = true;
-        bool again = false;
-        if (use_cand)
-            for (int b = 0; b < (iss + c->batch - 1) / c->batch; b++)
-                if (cnt[1 + iss + b] > c->cand_cap) again = true;
-        const long long ntot = cnt[0];
-        if (ntot > c->trig_cap) again = true;
-        if (again || overflow) {
-            if (!overflow) redo.push_back(j);
-            return TDF_OK;
+        for (int k = 0; k < gsize(gi); k++) {
+            const int j = grp(gi, k);
+            S[j].aa = c->scal_hb[slot][4 * k];
+            S[j].bb = c->scal_hb[slot][4 * k + 1];
+            skies[j] = S[j];
+            const int iss = S[j].iss_size;
+            if (iss <= 0) continue;
+            const int *cnt = c->cnt_hb[slot] + (size_t)k * c->cnt_hb_cap;
+            for (int t = 0; t < iss; t++)
+                if (cnt[1 + t] > maxtrig) overflow = true;
+            bool again = cnt[0] > prefix;
+            if (use_cand)
+                for (int b = 0; b < (iss + c->batch - 1) / c->batch; b++)
+                    if (cnt[1 + iss + b] > c->cand_cap) again = true;
+            if (overflow) return TDF_OK;
+            if (again) {
+                redo.push_back(j);
+                continue;
+            }
+            nrec_per[j] = iss;
+            int r = fill_records(c, S[j], mm[j], nn[j], o, iss, c->trig_hb[slot] + (size_t)k * BATCH_TRIG_PREFIX,
+                                 cnt[0], recs + roff[j], pairs, pair_cap, &np);
+            if (r) return r;
         }
-        nrec_per[j] = iss;
-        if (ntot <= prefix)
-            return fill_records(c, S[j], mm[j], nn[j], o, iss, c->trig_hb[slot], ntot, recs + roff[j], pairs,
-                                pair_cap, &np);
-        // the rest of a long list: this slot's device list is not written again
-        // before the next use of the slot, which is enqueued after this call
-        std::vector<TrigEntry> all((size_t)ntot);
-        std::copy(c->trig_hb[slot], c->trig_hb[slot] + prefix, all.begin());
-        HIPCHK(hipMemcpy(all.data() + prefix, c->trig_ds[slot] + prefix, sizeof(TrigEntry) * (ntot - prefix),
-                         hipMemcpyDeviceToHost));
-        return fill_records(c, S[j], mm[j], nn[j], o, iss, all.data(), ntot, recs + roff[j], pairs, pair_cap,
-                            &np);
+        return TDF_OK;
     };
 
-    const int n = (int)vis.size();
-    for (int idx = 0; idx <= n + 1 && !rc && !overflow; idx++) {
-        if (idx >= 1 && idx <= n) rc = enqueue_spin(vis[idx - 1], (idx - 1) % 2);
-        if (!rc && idx >= 2) rc = finish(vis[idx - 2], idx % 2);
-        if (!rc && idx < n) rc = enqueue_sky(vis[idx], idx % 2);
+    for (int idx = 0; idx <= ngrp + 1 && !rc && !overflow; idx++) {
+        if (idx >= 1 && idx <= ngrp) rc = enqueue_spin(idx - 1, (idx - 1) % 2);
+        if (!rc && idx >= 2) rc = finish(idx - 2, idx % 2);
+        if (!rc && idx < ngrp) rc = enqueue_sky(idx, idx % 2);
     }
     (void)hipStreamSynchronize(c->sstream);
     (void)hipStreamSynchronize(c->stream);
-    use_slot(c, 0);
+    c->xm = xm0;
+    c->tmp1 = tmp10;
+    c->sky_scal = scal0;
     c->sky_valid = false;
     prof_collect(c);
     if (rc) return rc;
     if (overflow) return TDF_ERR_TRIGGER_OVERFLOW;
-    // points whose triggers did not fit the pipelined readback: synchronous path
+    // points whose triggers did not fit the pipelined read-back: synchronous path
     for (int j : redo) {
         tdf_sky sj;
         if ((rc = tdf_sky_stage(c, pm[j], mm[j], nn[j], o, &sj))) return rc;

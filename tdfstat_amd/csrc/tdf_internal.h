@@ -56,19 +56,27 @@ struct SpinLayout {
     __host__ __device__ inline size_t at(int i) const { return (size_t)(i & (n1 - 1)) * n2 + (i >> l1); }
 };
 
+// Per-sky constants of the sky stage (jobcore.c:242-283, settings.c:265-272).
+struct SkySite {
+    double het0;
+    double ns0, ns1, ns2;  // unit vector to the source
+    double c2d, c2sd, sindel, cosdel;
+    double cosalfr[MAXDET], sinalfr[MAXDET];
+};
+
+// Sky stage of a group of G sky points (grid z / item index = point g);
+// per-point outputs at the strides noted.
 struct SkyPrepArgs {
     SpinLayout lay;
     const float *xDat;      // [nifo][N]
     const double *DetSSB;   // [nifo][3N]
     const DetParams *det;   // [nifo]
-    double *shftf;          // [nifo][N] out
-    double *tmp1;           // [nifo][nfftf] out, SpinLayout order
-    double *partials;       // [nifo][2][nblk] out
+    const SkySite *site;    // [G] (device)
+    double *shftf;          // [G][nifo][N] out
+    double *tmp1;           // [G][nifo][nfftf] out, SpinLayout order
+    double *partials;       // [G][nifo][2][nblk] out
     int N, nfft;
-    double omr, oms, het0;
-    double ns0, ns1, ns2;
-    double c2d, c2sd, sindel, cosdel;
-    double cosalfr[MAXDET], sinalfr[MAXDET];
+    double omr, oms;
 };
 
 // Sky-dependent constants of modvir (settings.c:254-294) for one detector.
@@ -206,8 +214,8 @@ template <typename T>
 hipError_t launch_trigger_sums(const SumArgs &a, hipStream_t st);
 
 template <typename T>
-hipError_t launch_sky_prep(const SkyPrepArgs &a, void *xab, int nblk, int nifo, hipStream_t st);
-hipError_t launch_sky_reduce(const double *partials, int nblk, int nifo, const double *sig2, double *out,
+hipError_t launch_sky_prep(const SkyPrepArgs &a, void *xab, int nblk, int nifo, int G, hipStream_t st);
+hipError_t launch_sky_reduce(const double *partials, int nblk, int nifo, const double *sig2, double *out, int G,
                              hipStream_t st);
 template <typename T>
 hipError_t launch_fft_natural_to_T(const FftPlan &p, const void *in, void *tmp, void *outT, int npairs, int sign,
@@ -220,7 +228,7 @@ hipError_t launch_spline(const void *Y, void *Y2, int n, size_t ystride, size_t 
                          const double *taps, hipStream_t st);
 template <typename T>
 hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, const double *sig2, void *xm, int N,
-                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, SpinLayout lay,
+                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, int nifo, SpinLayout lay,
                          size_t xmstride, hipStream_t st);
 template <typename T>
 hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st);

@@ -39,31 +39,42 @@ __device__ __forceinline__ double tmp1_of(int i, double shft)
 #pragma clang fp contract(off)
     return (double)i * (double)i + (double)(2 * i) * shft;
 }
+// first demodulation phase het0*i + oms*shft with separate roundings
+// (jobcore.c:306)
+__device__ __forceinline__ double phase0_of(double het0, int i, double oms, double shft)
+{
+#pragma clang fp contract(off)
+    return het0 * (double)i + oms * shft;
+}
 
-// Per-sky prep: one thread per sample i < nfft of detector blockIdx.y.
+// Per-sky prep: one thread per sample i < nfft of detector blockIdx.y of
+// sky point blockIdx.z of the group.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restrict__ xab)
 {
-    const int d = blockIdx.y;
+    const int d = blockIdx.y, g = blockIdx.z, nifo = gridDim.y;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const DetParams &dp = a.det[d];
+    const SkySite &ss = a.site[g];
+    const size_t nfftf = (size_t)a.lay.n1 * a.lay.n2;
+    xab += (size_t)g * 2 * nifo * a.nfft;
     double a2 = 0., b2 = 0.;
     if (i < a.nfft) {
         cx<T> xa = {(T)0, (T)0}, xb = {(T)0, (T)0};
         if (i < a.N) {
             const float x = a.xDat[(size_t)d * a.N + i];
             const double *ssb0 = a.DetSSB + (size_t)d * 3 * a.N;
-            const double shft = shft_dot(a.ns0, a.ns1, a.ns2, ssb0 + 3 * (size_t)i);
-            const double shftf = shft - shft_dot(a.ns0, a.ns1, a.ns2, ssb0);
+            const double shft = shft_dot(ss.ns0, ss.ns1, ss.ns2, ssb0 + 3 * (size_t)i);
+            const double shftf = shft - shft_dot(ss.ns0, ss.ns1, ss.ns2, ssb0);
             const double tmp1 = tmp1_of(i, shft);
-            a.shftf[(size_t)d * a.N + i] = shftf;
-            a.tmp1[(size_t)d * a.lay.n1 * a.lay.n2 + a.lay.at(i)] = tmp1;
+            a.shftf[((size_t)g * nifo + d) * a.N + i] = shftf;
+            a.tmp1[((size_t)g * nifo + d) * nfftf + a.lay.at(i)] = tmp1;
             double aa, bb;
-            const ModvirSky ms{a.c2d, a.c2sd, a.sindel, a.cosdel, a.cosalfr[d], a.sinalfr[d]};
+            const ModvirSky ms{ss.c2d, ss.c2sd, ss.sindel, ss.cosdel, ss.cosalfr[d], ss.sinalfr[d]};
             modvir_sample(dp.c, ms, a.omr, i, x, &aa, &bb);
             a2 = aa * aa;
             b2 = bb * bb;
-            const double phase = a.het0 * i + a.oms * shft;
+            const double phase = phase0_of(ss.het0, i, a.oms, shft);
             T sp, cp;
             sincos_fp64arg(phase, &sp, &cp);
             const T xaa = (T)((double)x * aa), xbb = (T)((double)x * bb);
@@ -86,16 +97,20 @@ __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restri
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        a.partials[((size_t)d * 2 + 0) * gridDim.x + blockIdx.x] = ra[0];
-        a.partials[((size_t)d * 2 + 1) * gridDim.x + blockIdx.x] = rb[0];
+        double *pg = a.partials + (size_t)g * nifo * 2 * gridDim.x;
+        pg[((size_t)d * 2 + 0) * gridDim.x + blockIdx.x] = ra[0];
+        pg[((size_t)d * 2 + 1) * gridDim.x + blockIdx.x] = rb[0];
     }
 }
 
-// aa = sum_d (sum_i aa_d^2)/sig2_d, likewise bb (jobcore.c:373-391)
+// aa = sum_d (sum_i aa_d^2)/sig2_d, likewise bb (jobcore.c:373-391); one
+// workgroup per sky point of the group, out[4g], out[4g+1].
 __global__ void __launch_bounds__(256) k_sky_reduce(const double *__restrict__ partials, int nblk,
                                                     int nifo, const double *__restrict__ sig2,
                                                     double *__restrict__ out)
 {
+    partials += (size_t)blockIdx.x * nifo * 2 * nblk;
+    out += 4 * (size_t)blockIdx.x;
     __shared__ double r[2][256];
     double acc[2] = {0., 0.};
     for (int d = 0; d < nifo; d++) {
@@ -262,6 +277,7 @@ struct IO_SpinP1 {
         if (nzero > 0 && blockIdx.x == 0 && blockIdx.y == 0)
             for (int k = threadIdx.x; k < nzero; k += NT) zero_cnt[k] = 0;
         const TmplDesc td = tmpl_scalars(sk, sk.iss0 + slot0 + blockIdx.y);
+        const double hr = td.het1 * 0.15915494309189533577, sr = td.spnd * 0.15915494309189533577;
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         const int nd = NIFO > 0 ? NIFO : nifo;
         // inputs are column-major (column c is contiguous over q): lanes
@@ -274,13 +290,20 @@ struct IO_SpinP1 {
             const size_t t = (size_t)(c0 + cl) * L + q;
             cx<T> fa = {(T)0, (T)0}, fb = {(T)0, (T)0};
             if (i < N) {
-                const double ph0 = td.het1 * (double)i;
+                // FP32 bulk: the phase is formed directly in turns (het1/2pi,
+                // spnd/2pi), reduced in FP64, sin/cos in hardware; FP64 path:
+                // the phase in radians as the reference forms it
+                const double ph0 = (sizeof(T) == 4 ? hr : td.het1) * (double)i;
 #pragma unroll
                 for (int d = 0; d < (NIFO > 0 ? NIFO : MAXDET); d++) {
                     if (NIFO == 0 && d >= nd) break;
-                    const double ph = ph0 + td.spnd * tmp1[(size_t)d * n + t];
                     T s, c;
-                    sincos_fp64arg(ph, &s, &c);
+                    if constexpr (sizeof(T) == 4) {
+                        sincos_rev(fma(sr, tmp1[(size_t)d * n + t], ph0), &s, &c);
+                    } else {
+                        const double ph = ph0 + td.spnd * tmp1[(size_t)d * n + t];
+                        sincos_fp64arg(ph, &s, &c);
+                    }
                     const cx<T> ma = xm[(size_t)(2 * d) * n + t];
                     const cx<T> mb = xm[(size_t)(2 * d + 1) * n + t];
                     fa.x += ma.x * c + ma.y * s;
@@ -539,12 +562,13 @@ __global__ void __launch_bounds__(256) k_splint(const cx<T> *__restrict__ Y, con
                                                 const double *__restrict__ shftf,
                                                 const double *__restrict__ sig2, cx<T> *__restrict__ xm,
                                                 int N, int n, int ipad, size_t ystride,
-                                                size_t y2stride, SpinLayout lay, size_t xmstride)
+                                                size_t y2stride, int nifo, SpinLayout lay, size_t xmstride)
 {
-    const int item = blockIdx.y, d = item >> 1;
+    // item = 2*(g*nifo + d) + (0: a, 1: b) for sky point g of the group
+    const int item = blockIdx.y, gd = item >> 1, d = gd % nifo;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
-    const double x = ipad * (i - shftf[(size_t)d * N + i]);
+    const double x = ipad * (i - shftf[(size_t)gd * N + i]);
     cx<T> out = {(T)0, (T)0};
     if (!(x < 0 || x > n - 1)) {
         const int klo = (int)floor(x);
@@ -872,16 +896,16 @@ static hipError_t launch_pass(int L, int E, IO io, dim3 grid, const cx<T> *twL, 
 }
 
 template <typename T>
-hipError_t launch_sky_prep(const SkyPrepArgs &a, void *xab, int nblk, int nifo, hipStream_t st)
+hipError_t launch_sky_prep(const SkyPrepArgs &a, void *xab, int nblk, int nifo, int G, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sky_prep<T>, dim3(nblk, nifo), dim3(256), 0, st, a, (cx<T> *)xab);
+    hipLaunchKernelGGL(k_sky_prep<T>, dim3(nblk, nifo, G), dim3(256), 0, st, a, (cx<T> *)xab);
     return hipGetLastError();
 }
 
-hipError_t launch_sky_reduce(const double *partials, int nblk, int nifo, const double *sig2, double *out,
+hipError_t launch_sky_reduce(const double *partials, int nblk, int nifo, const double *sig2, double *out, int G,
                              hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sky_reduce, dim3(1), dim3(256), 0, st, partials, nblk, nifo, sig2, out);
+    hipLaunchKernelGGL(k_sky_reduce, dim3(G), dim3(256), 0, st, partials, nblk, nifo, sig2, out);
     return hipGetLastError();
 }
 
@@ -927,11 +951,11 @@ hipError_t launch_spline(const void *Y, void *Y2, int n, size_t ystride, size_t 
 
 template <typename T>
 hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, const double *sig2, void *xm, int N,
-                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, SpinLayout lay,
+                         int n, int ipad, size_t ystride, size_t y2stride, int nitems, int nifo, SpinLayout lay,
                          size_t xmstride, hipStream_t st)
 {
     hipLaunchKernelGGL(k_splint<T>, dim3((N + 255) / 256, nitems), dim3(256), 0, st, (const cx<T> *)Y,
-                       (const cx<T> *)Y2, shftf, sig2, (cx<T> *)xm, N, n, ipad, ystride, y2stride, lay,
+                       (const cx<T> *)Y2, shftf, sig2, (cx<T> *)xm, N, n, ipad, ystride, y2stride, nifo, lay,
                        xmstride);
     return hipGetLastError();
 }
@@ -1030,7 +1054,7 @@ hipError_t launch_candidates(const void *FT, const TrigArgs &a, const TrigEntry 
 }
 
 #define TDF_INST(T)                                                                                        \
-    template hipError_t launch_sky_prep<T>(const SkyPrepArgs &, void *, int, int, hipStream_t);           \
+    template hipError_t launch_sky_prep<T>(const SkyPrepArgs &, void *, int, int, int, hipStream_t);      \
     template hipError_t launch_fft_natural_to_T<T>(const FftPlan &, const void *, void *, void *, int, int, \
                                                    int, int, double, hipStream_t);                         \
     template hipError_t launch_transpose<T>(const FftPlan &, const void *, void *, int, size_t, size_t, int,  \
@@ -1038,7 +1062,7 @@ hipError_t launch_candidates(const void *FT, const TrigArgs &a, const TrigEntry 
     template hipError_t launch_spline<T>(const void *, void *, int, size_t, size_t, int, const double *,     \
                                          hipStream_t);                                                     \
     template hipError_t launch_splint<T>(const void *, const void *, const double *, const double *, void *, \
-                                         int, int, int, size_t, size_t, int, SpinLayout, size_t,           \
+                                         int, int, int, size_t, size_t, int, int, SpinLayout, size_t,      \
                                          hipStream_t);                                                     \
     template hipError_t launch_spin_pass1<T>(const FftPlan &, const SpinArgs &, hipStream_t);              \
     template hipError_t launch_spin_pass2<T>(const FftPlan &, const SpinArgs &, hipStream_t);              \
