@@ -138,9 +138,10 @@ int tdf_job_core(tdf_ctx *ctx, int pm, float mm, float nn,
                  long long *npairs);
 
 /* The sky loop of search() (jobcore.c:88-111) over nsky points
- * (pm[k], mm[k], nn[k]) in one call: job_core for each, pipelined on two
- * streams (the per-sky stage of a point runs while the spindown loop of the
- * previous point runs; trigger read-back overlaps the next point's work).
+ * (pm[k], mm[k], nn[k]) in one call: job_core for each, in groups of sky
+ * points (one launch sequence runs the per-sky stage of a whole group, then
+ * the group's spindown loops; one read-back per group, turned into records
+ * on the host while the device runs the next group).
  * skies[k] receive each point's scalars and nrec_per[k] its record count
  * (iss_size, 0 if not visible); records are written consecutively in input
  * order, (f, F) pairs into pairs.  *nrec / *npairs return the totals.  The

@@ -112,15 +112,17 @@ struct tdf_ctx {
     int last_fr0 = 0, last_fr1 = -1, last_nblk = 0, last_nav = 1;
     // per-sky constants: [0] single-point path, [1 + slot*GMAX + k] groups
     SkySite *site_d = nullptr, *site_h = nullptr;
-    // multi-sky pipeline (tdf_job_core_batch): sky stage of a group of gmax
-    // points into one of two slots, sky stream, events, pinned read-back
+    // multi-sky pipeline (tdf_job_core_batch): sky stage of a group of up to
+    // gmax points, its spindown loops, one read-back per group into one of
+    // two pinned slots (the host turns group g into records while the
+    // device runs group g+1)
     int gmax = 0;
-    hipStream_t sstream = nullptr;
+    bool batch_ready = false;
     void *xab_g = nullptr, *Zsky_g = nullptr, *XT_g = nullptr, *Ymv_g = nullptr;
     double *shftf_g = nullptr, *partials_g = nullptr;
     void *xm_g[2] = {nullptr, nullptr};
     double *tmp1_g[2] = {nullptr, nullptr}, *scal_g[2] = {nullptr, nullptr};
-    hipEvent_t evS[2] = {nullptr, nullptr}, evA[2] = {nullptr, nullptr};
+    hipEvent_t evA[2] = {nullptr, nullptr};
     int *cnt_hb[2] = {nullptr, nullptr}, *cnt_g[2] = {nullptr, nullptr};  // host / device, gmax x cnt_hb_cap
     TrigEntry *trig_hb[2] = {nullptr, nullptr}, *trig_g[2] = {nullptr, nullptr};  // gmax x prefix
     double *scal_hb[2] = {nullptr, nullptr};
@@ -551,7 +553,6 @@ int tdf_ctx_destroy(tdf_ctx *c)
 {
     if (!c) return TDF_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->sstream) (void)hipStreamSynchronize(c->sstream);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->cnt_d) (void)hipFree(c->cnt_d);
     if (c->trig_h) (void)hipHostFree(c->trig_h);
@@ -562,10 +563,8 @@ int tdf_ctx_destroy(tdf_ctx *c)
         if (c->cnt_g[k]) (void)hipFree(c->cnt_g[k]);
         if (c->trig_hb[k]) (void)hipHostFree(c->trig_hb[k]);
         if (c->scal_hb[k]) (void)hipHostFree(c->scal_hb[k]);
-        if (c->evS[k]) (void)hipEventDestroy(c->evS[k]);
         if (c->evA[k]) (void)hipEventDestroy(c->evA[k]);
     }
-    if (c->sstream) (void)hipStreamDestroy(c->sstream);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -857,11 +856,11 @@ int ensure_batch(tdf_ctx *c, int ncnt)
     const tdf_geom &g = c->g;
     const size_t cs = c->csz;
     const int nifo = g.nifo;
-    if (!c->sstream) {
+    if (!c->batch_ready) {
         const size_t per = cs * 2 * nifo * ((size_t)g.nfft + 3 * (size_t)g.Ninterp) + 8 * nifo * (size_t)g.N;
         c->gmax = (int)std::max<size_t>(1, std::min<size_t>(GMAX_SKY, (512ull << 20) / per));
         const size_t G = c->gmax;
-        HIPCHK(hipStreamCreateWithFlags(&c->sstream, hipStreamNonBlocking));
+        c->batch_ready = true;
         int rc;
         if ((rc = dmalloc(c, &c->xab_g, G * cs * 2 * nifo * (size_t)g.nfft))) return rc;
         if ((rc = dmalloc(c, &c->Zsky_g, G * cs * 2 * nifo * (size_t)g.Ninterp))) return rc;
@@ -870,19 +869,16 @@ int ensure_batch(tdf_ctx *c, int ncnt)
         if ((rc = dmalloc(c, (void **)&c->shftf_g, G * sizeof(double) * nifo * (size_t)g.N))) return rc;
         if ((rc = dmalloc(c, (void **)&c->partials_g, G * sizeof(double) * 2 * nifo * (size_t)c->prep_nblk)))
             return rc;
+        // device buffers: one set (the stream orders the groups)
+        if ((rc = dmalloc(c, &c->xm_g[0], G * cs * 2 * nifo * (size_t)g.nfftf))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->tmp1_g[0], G * sizeof(double) * nifo * (size_t)g.nfftf))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->scal_g[0], G * sizeof(double) * 4))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->trig_g[0], G * sizeof(TrigEntry) * BATCH_TRIG_PREFIX))) return rc;
+        HIPCHK(hipMemset(c->xm_g[0], 0, G * cs * 2 * nifo * (size_t)g.nfftf));
+        HIPCHK(hipMemset(c->tmp1_g[0], 0, G * sizeof(double) * nifo * (size_t)g.nfftf));
+        // host read-back: two slots
         for (int k = 0; k < 2; k++) {
-            if ((rc = dmalloc(c, &c->xm_g[k], G * cs * 2 * nifo * (size_t)g.nfftf))) return rc;
-            if ((rc = dmalloc(c, (void **)&c->tmp1_g[k], G * sizeof(double) * nifo * (size_t)g.nfftf))) return rc;
-            if ((rc = dmalloc(c, (void **)&c->scal_g[k], G * sizeof(double) * 4))) return rc;
-            // padding beyond N is read by the spindown kernels: keep it zero
-            HIPCHK(hipMemset(c->xm_g[k], 0, G * cs * 2 * nifo * (size_t)g.nfftf));
-            HIPCHK(hipMemset(c->tmp1_g[k], 0, G * sizeof(double) * nifo * (size_t)g.nfftf));
-            HIPCHK(hipEventCreateWithFlags(&c->evS[k], hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&c->evA[k], hipEventDisableTiming));
-            // recorded once so that the first waits on them are satisfied
-            HIPCHK(hipEventRecord(c->evS[k], c->sstream));
-            HIPCHK(hipEventRecord(c->evA[k], c->stream));
-            if ((rc = dmalloc(c, (void **)&c->trig_g[k], G * sizeof(TrigEntry) * BATCH_TRIG_PREFIX))) return rc;
             if (hipHostMalloc((void **)&c->trig_hb[k], G * sizeof(TrigEntry) * BATCH_TRIG_PREFIX, 0) != hipSuccess ||
                 hipHostMalloc((void **)&c->scal_hb[k], G * sizeof(double) * 4, 0) != hipSuccess)
                 return TDF_ERR_NOMEM;
@@ -890,13 +886,13 @@ int ensure_batch(tdf_ctx *c, int ncnt)
     }
     if (ncnt > c->cnt_hb_cap) {
         HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->cnt_g[0]) (void)hipFree(c->cnt_g[0]);
+        c->cnt_g[0] = nullptr;
+        if (hipMalloc((void **)&c->cnt_g[0], sizeof(int) * ncnt * (size_t)c->gmax) != hipSuccess) return TDF_ERR_NOMEM;
         for (int k = 0; k < 2; k++) {
             if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
-            if (c->cnt_g[k]) (void)hipFree(c->cnt_g[k]);
             c->cnt_hb[k] = nullptr;
-            c->cnt_g[k] = nullptr;
-            if (hipHostMalloc((void **)&c->cnt_hb[k], sizeof(int) * ncnt * (size_t)c->gmax, 0) != hipSuccess ||
-                hipMalloc((void **)&c->cnt_g[k], sizeof(int) * ncnt * (size_t)c->gmax) != hipSuccess)
+            if (hipHostMalloc((void **)&c->cnt_hb[k], sizeof(int) * ncnt * (size_t)c->gmax, 0) != hipSuccess)
                 return TDF_ERR_NOMEM;
         }
         c->cnt_hb_cap = ncnt;
@@ -906,17 +902,17 @@ int ensure_batch(tdf_ctx *c, int ncnt)
 
 SkyBufs group_bufs(tdf_ctx *c, int slot)
 {
-    return SkyBufs{c->xab_g, c->Zsky_g, c->XT_g, c->Ymv_g, c->xm_g[slot], c->shftf_g, c->partials_g,
-                   c->tmp1_g[slot], c->scal_g[slot], c->site_d + 1 + slot * GMAX_SKY};
+    return SkyBufs{c->xab_g, c->Zsky_g, c->XT_g, c->Ymv_g, c->xm_g[0], c->shftf_g, c->partials_g,
+                   c->tmp1_g[0], c->scal_g[0], c->site_d + 1 + slot * GMAX_SKY};
 }
 
-// point k of the group in slot: its spindown inputs become the current ones
-void use_point(tdf_ctx *c, int slot, int k)
+// point k of the current group: its spindown inputs become the current ones
+void use_point(tdf_ctx *c, int k)
 {
     const tdf_geom &g = c->g;
-    c->xm = (char *)c->xm_g[slot] + (size_t)k * c->csz * 2 * g.nifo * g.nfftf;
-    c->tmp1 = c->tmp1_g[slot] + (size_t)k * g.nifo * g.nfftf;
-    c->sky_scal = c->scal_g[slot] + 4 * (size_t)k;
+    c->xm = (char *)c->xm_g[0] + (size_t)k * c->csz * 2 * g.nifo * g.nfftf;
+    c->tmp1 = c->tmp1_g[0] + (size_t)k * g.nifo * g.nfftf;
+    c->sky_scal = c->scal_g[0] + 4 * (size_t)k;
 }
 
 // Records of one spindown call from its trigger list (ascending bin per
@@ -996,10 +992,11 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
                               hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    // groups of G visible points: at least three groups per call when there
-    // are enough points, so that sky stages overlap spindown loops
+    // groups of G visible points: at least two groups per call when there
+    // are enough points, so that the host work of one group overlaps the
+    // device work of the next
     const int nv = (int)vis.size();
-    const int G = std::max(1, std::min(c->gmax, (nv + 2) / 3));
+    const int G = std::max(1, std::min(c->gmax, (nv + 1) / 2));
     const int ngrp = (nv + G - 1) / G;
     const bool use_cand = !o->fstat_norm || (o->fr1 - o->fr0) / std::max(o->nav, 1) == 0;
     const int maxtrig = o->bufsize / 2;
@@ -1013,40 +1010,33 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
     auto grp = [&](int gi, int k) { return vis[(size_t)gi * G + k]; };
     auto gsize = [&](int gi) { return std::min(G, nv - gi * G); };
 
-    auto enqueue_sky = [&](int gi, int slot) -> int {
+    // group gi: sky stage, spindown loops, one read-back into host slot
+    auto enqueue_group = [&](int gi, int slot) -> int {
         const int ng = gsize(gi);
         SkySite *sh = c->site_h + 1 + slot * GMAX_SKY;
-        HIPCHK(hipStreamWaitEvent(c->sstream, c->evA[slot], 0));
         for (int k = 0; k < ng; k++) sh[k] = site[grp(gi, k)];
         const SkyBufs b = group_bufs(c, slot);
-        HIPCHK(hipMemcpyAsync((void *)b.site, sh, sizeof(SkySite) * ng, hipMemcpyHostToDevice, c->sstream));
-        int r = run_sky_stage<T>(c, b, ng, c->sstream);
+        HIPCHK(hipMemcpyAsync((void *)b.site, sh, sizeof(SkySite) * ng, hipMemcpyHostToDevice, c->stream));
+        int r = run_sky_stage<T>(c, b, ng, c->stream);
         if (r) return r;
-        HIPCHK(hipMemcpyAsync(c->scal_hb[slot], c->scal_g[slot], sizeof(double) * 4 * ng, hipMemcpyDeviceToHost,
-                              c->sstream));
-        HIPCHK(hipEventRecord(c->evS[slot], c->sstream));
-        return TDF_OK;
-    };
-    auto enqueue_spin = [&](int gi, int slot) -> int {
-        HIPCHK(hipStreamWaitEvent(c->stream, c->evS[slot], 0));
-        const int ng = gsize(gi);
         for (int k = 0; k < ng; k++) {
             const int j = grp(gi, k);
             const int iss = S[j].iss_size;
             if (iss <= 0) continue;
-            use_point(c, slot, k);
+            use_point(c, k);
             const SkyScal sk{S[j].smin, o->sstep, mm[j], nn[j], S[j].het0, M[0], M[4], M[5], M[9], M[13], 0};
             const int nc = 1 + iss + (iss + c->batch - 1) / c->batch;
             // per-point counters and trigger list in the group's device area,
             // counters cleared by the first pass-1 launch (no memset call)
-            int r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_g[slot] + (size_t)k * c->cnt_hb_cap,
-                                             iss, nc, c->trig_g[slot] + (size_t)k * BATCH_TRIG_PREFIX, prefix);
+            r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_g[0] + (size_t)k * c->cnt_hb_cap, iss, nc,
+                                         c->trig_g[0] + (size_t)k * BATCH_TRIG_PREFIX, prefix);
             if (r) return r;
         }
-        // one read-back per group
-        HIPCHK(hipMemcpyAsync(c->cnt_hb[slot], c->cnt_g[slot], sizeof(int) * ng * (size_t)c->cnt_hb_cap,
+        HIPCHK(hipMemcpyAsync(c->scal_hb[slot], c->scal_g[0], sizeof(double) * 4 * ng, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->cnt_hb[slot], c->cnt_g[0], sizeof(int) * ng * (size_t)c->cnt_hb_cap,
                               hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(c->trig_hb[slot], c->trig_g[slot], sizeof(TrigEntry) * ng * BATCH_TRIG_PREFIX,
+        HIPCHK(hipMemcpyAsync(c->trig_hb[slot], c->trig_g[0], sizeof(TrigEntry) * ng * BATCH_TRIG_PREFIX,
                               hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipEventRecord(c->evA[slot], c->stream));
         return TDF_OK;
@@ -1080,12 +1070,10 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
         return TDF_OK;
     };
 
-    for (int idx = 0; idx <= ngrp + 1 && !rc && !overflow; idx++) {
-        if (idx >= 1 && idx <= ngrp) rc = enqueue_spin(idx - 1, (idx - 1) % 2);
-        if (!rc && idx >= 2) rc = finish(idx - 2, idx % 2);
-        if (!rc && idx < ngrp) rc = enqueue_sky(idx, idx % 2);
+    for (int idx = 0; idx <= ngrp && !rc && !overflow; idx++) {
+        if (idx < ngrp) rc = enqueue_group(idx, idx % 2);
+        if (!rc && idx >= 1) rc = finish(idx - 1, (idx - 1) % 2);
     }
-    (void)hipStreamSynchronize(c->sstream);
     (void)hipStreamSynchronize(c->stream);
     c->xm = xm0;
     c->tmp1 = tmp10;
