@@ -120,12 +120,13 @@ struct tdf_ctx {
     bool batch_ready = false;
     void *xab_g = nullptr, *Zsky_g = nullptr, *XT_g = nullptr, *Ymv_g = nullptr;
     double *shftf_g = nullptr, *partials_g = nullptr;
-    void *xm_g[2] = {nullptr, nullptr};
-    double *tmp1_g[2] = {nullptr, nullptr}, *scal_g[2] = {nullptr, nullptr};
+    void *xm_g = nullptr;
+    double *tmp1_g = nullptr;
     hipEvent_t evA[2] = {nullptr, nullptr};
-    int *cnt_hb[2] = {nullptr, nullptr}, *cnt_g[2] = {nullptr, nullptr};  // host / device, gmax x cnt_hb_cap
-    TrigEntry *trig_hb[2] = {nullptr, nullptr}, *trig_g[2] = {nullptr, nullptr};  // gmax x prefix
-    double *scal_hb[2] = {nullptr, nullptr};
+    // read-back block of a group, device and two pinned host copies:
+    // [aa, bb, -, - per point | counters: gmax x cnt_hb_cap | triggers: gmax x prefix]
+    char *stage_d = nullptr, *stage_h[2] = {nullptr, nullptr};
+    size_t stage_bytes = 0;
     int cnt_hb_cap = 0;
     // profiling: bit 0 spindown kernels, bit 1 per-sky stage kernels
     int prof = 0;
@@ -558,11 +559,9 @@ int tdf_ctx_destroy(tdf_ctx *c)
     if (c->trig_h) (void)hipHostFree(c->trig_h);
     if (c->cnt_h) (void)hipHostFree(c->cnt_h);
     if (c->site_h) (void)hipHostFree(c->site_h);
+    if (c->stage_d) (void)hipFree(c->stage_d);
     for (int k = 0; k < 2; k++) {
-        if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
-        if (c->cnt_g[k]) (void)hipFree(c->cnt_g[k]);
-        if (c->trig_hb[k]) (void)hipHostFree(c->trig_hb[k]);
-        if (c->scal_hb[k]) (void)hipHostFree(c->scal_hb[k]);
+        if (c->stage_h[k]) (void)hipHostFree(c->stage_h[k]);
         if (c->evA[k]) (void)hipEventDestroy(c->evA[k]);
     }
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -846,11 +845,11 @@ int tdf_spindown(tdf_ctx *c, int iss_begin, int iss_count, const tdf_job_opts *o
 
 namespace {
 
-constexpr int BATCH_TRIG_PREFIX = 4096;  // triggers read back per sky point without a sync
+constexpr int BATCH_TRIG_PREFIX = 1024;  // triggers per sky point read back with the group
 
-// Group sky-stage buffers (gmax points), two slots of per-point spindown
-// inputs, the sky stream, events and pinned read-back; allocated on first
-// use.  gmax: as many points as fit 512 MB of sky-stage buffers, <= 16.
+// Group sky-stage buffers (gmax points), the group's spindown inputs, the
+// read-back block and events; allocated on first use.  gmax: as many points
+// as fit 512 MB of sky-stage buffers, <= 16.
 int ensure_batch(tdf_ctx *c, int ncnt)
 {
     const tdf_geom &g = c->g;
@@ -860,7 +859,6 @@ int ensure_batch(tdf_ctx *c, int ncnt)
         const size_t per = cs * 2 * nifo * ((size_t)g.nfft + 3 * (size_t)g.Ninterp) + 8 * nifo * (size_t)g.N;
         c->gmax = (int)std::max<size_t>(1, std::min<size_t>(GMAX_SKY, (512ull << 20) / per));
         const size_t G = c->gmax;
-        c->batch_ready = true;
         int rc;
         if ((rc = dmalloc(c, &c->xab_g, G * cs * 2 * nifo * (size_t)g.nfft))) return rc;
         if ((rc = dmalloc(c, &c->Zsky_g, G * cs * 2 * nifo * (size_t)g.Ninterp))) return rc;
@@ -869,50 +867,55 @@ int ensure_batch(tdf_ctx *c, int ncnt)
         if ((rc = dmalloc(c, (void **)&c->shftf_g, G * sizeof(double) * nifo * (size_t)g.N))) return rc;
         if ((rc = dmalloc(c, (void **)&c->partials_g, G * sizeof(double) * 2 * nifo * (size_t)c->prep_nblk)))
             return rc;
-        // device buffers: one set (the stream orders the groups)
-        if ((rc = dmalloc(c, &c->xm_g[0], G * cs * 2 * nifo * (size_t)g.nfftf))) return rc;
-        if ((rc = dmalloc(c, (void **)&c->tmp1_g[0], G * sizeof(double) * nifo * (size_t)g.nfftf))) return rc;
-        if ((rc = dmalloc(c, (void **)&c->scal_g[0], G * sizeof(double) * 4))) return rc;
-        if ((rc = dmalloc(c, (void **)&c->trig_g[0], G * sizeof(TrigEntry) * BATCH_TRIG_PREFIX))) return rc;
-        HIPCHK(hipMemset(c->xm_g[0], 0, G * cs * 2 * nifo * (size_t)g.nfftf));
-        HIPCHK(hipMemset(c->tmp1_g[0], 0, G * sizeof(double) * nifo * (size_t)g.nfftf));
-        // host read-back: two slots
-        for (int k = 0; k < 2; k++) {
-            HIPCHK(hipEventCreateWithFlags(&c->evA[k], hipEventDisableTiming));
-            if (hipHostMalloc((void **)&c->trig_hb[k], G * sizeof(TrigEntry) * BATCH_TRIG_PREFIX, 0) != hipSuccess ||
-                hipHostMalloc((void **)&c->scal_hb[k], G * sizeof(double) * 4, 0) != hipSuccess)
-                return TDF_ERR_NOMEM;
-        }
+        if ((rc = dmalloc(c, &c->xm_g, G * cs * 2 * nifo * (size_t)g.nfftf))) return rc;
+        if ((rc = dmalloc(c, (void **)&c->tmp1_g, G * sizeof(double) * nifo * (size_t)g.nfftf))) return rc;
+        HIPCHK(hipMemset(c->xm_g, 0, G * cs * 2 * nifo * (size_t)g.nfftf));
+        HIPCHK(hipMemset(c->tmp1_g, 0, G * sizeof(double) * nifo * (size_t)g.nfftf));
+        for (int k = 0; k < 2; k++) HIPCHK(hipEventCreateWithFlags(&c->evA[k], hipEventDisableTiming));
+        c->batch_ready = true;
     }
     if (ncnt > c->cnt_hb_cap) {
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (c->cnt_g[0]) (void)hipFree(c->cnt_g[0]);
-        c->cnt_g[0] = nullptr;
-        if (hipMalloc((void **)&c->cnt_g[0], sizeof(int) * ncnt * (size_t)c->gmax) != hipSuccess) return TDF_ERR_NOMEM;
+        if (c->stage_d) (void)hipFree(c->stage_d);
+        c->stage_d = nullptr;
         for (int k = 0; k < 2; k++) {
-            if (c->cnt_hb[k]) (void)hipHostFree(c->cnt_hb[k]);
-            c->cnt_hb[k] = nullptr;
-            if (hipHostMalloc((void **)&c->cnt_hb[k], sizeof(int) * ncnt * (size_t)c->gmax, 0) != hipSuccess)
-                return TDF_ERR_NOMEM;
+            if (c->stage_h[k]) (void)hipHostFree(c->stage_h[k]);
+            c->stage_h[k] = nullptr;
         }
+        const size_t G = c->gmax;
+        c->stage_bytes = G * 4 * sizeof(double) + G * ncnt * sizeof(int) + G * BATCH_TRIG_PREFIX * sizeof(TrigEntry);
+        c->stage_bytes = (c->stage_bytes + 16 + 255) & ~(size_t)255;
+        if (hipMalloc((void **)&c->stage_d, c->stage_bytes) != hipSuccess) return TDF_ERR_NOMEM;
+        for (int k = 0; k < 2; k++)
+            if (hipHostMalloc((void **)&c->stage_h[k], c->stage_bytes, 0) != hipSuccess) return TDF_ERR_NOMEM;
         c->cnt_hb_cap = ncnt;
     }
     return TDF_OK;
 }
 
+// parts of a read-back block (device or host)
+double *stage_scal(tdf_ctx *, char *blk) { return (double *)blk; }
+int *stage_cnt(tdf_ctx *c, char *blk) { return (int *)(blk + 4 * sizeof(double) * c->gmax); }
+TrigEntry *stage_trig(tdf_ctx *c, char *blk)
+{
+    size_t off = 4 * sizeof(double) * c->gmax + sizeof(int) * (size_t)c->gmax * c->cnt_hb_cap;
+    off = (off + 15) & ~(size_t)15;
+    return (TrigEntry *)(blk + off);
+}
+
 SkyBufs group_bufs(tdf_ctx *c, int slot)
 {
-    return SkyBufs{c->xab_g, c->Zsky_g, c->XT_g, c->Ymv_g, c->xm_g[0], c->shftf_g, c->partials_g,
-                   c->tmp1_g[0], c->scal_g[0], c->site_d + 1 + slot * GMAX_SKY};
+    return SkyBufs{c->xab_g, c->Zsky_g, c->XT_g, c->Ymv_g, c->xm_g, c->shftf_g, c->partials_g,
+                   c->tmp1_g, stage_scal(c, c->stage_d), c->site_d + 1 + slot * GMAX_SKY};
 }
 
 // point k of the current group: its spindown inputs become the current ones
 void use_point(tdf_ctx *c, int k)
 {
     const tdf_geom &g = c->g;
-    c->xm = (char *)c->xm_g[0] + (size_t)k * c->csz * 2 * g.nifo * g.nfftf;
-    c->tmp1 = c->tmp1_g[0] + (size_t)k * g.nifo * g.nfftf;
-    c->sky_scal = c->scal_g[0] + 4 * (size_t)k;
+    c->xm = (char *)c->xm_g + (size_t)k * c->csz * 2 * g.nifo * g.nfftf;
+    c->tmp1 = c->tmp1_g + (size_t)k * g.nifo * g.nfftf;
+    c->sky_scal = stage_scal(c, c->stage_d) + 4 * (size_t)k;
 }
 
 // Records of one spindown call from its trigger list (ascending bin per
@@ -1028,16 +1031,13 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
             const int nc = 1 + iss + (iss + c->batch - 1) / c->batch;
             // per-point counters and trigger list in the group's device area,
             // counters cleared by the first pass-1 launch (no memset call)
-            r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand, c->cnt_g[0] + (size_t)k * c->cnt_hb_cap, iss, nc,
-                                         c->trig_g[0] + (size_t)k * BATCH_TRIG_PREFIX, prefix);
+            r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand,
+                                         stage_cnt(c, c->stage_d) + (size_t)k * c->cnt_hb_cap, iss, nc,
+                                         stage_trig(c, c->stage_d) + (size_t)k * BATCH_TRIG_PREFIX, prefix);
             if (r) return r;
         }
-        HIPCHK(hipMemcpyAsync(c->scal_hb[slot], c->scal_g[0], sizeof(double) * 4 * ng, hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(c->cnt_hb[slot], c->cnt_g[0], sizeof(int) * ng * (size_t)c->cnt_hb_cap,
-                              hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(c->trig_hb[slot], c->trig_g[0], sizeof(TrigEntry) * ng * BATCH_TRIG_PREFIX,
-                              hipMemcpyDeviceToHost, c->stream));
+        // aa/bb, counters and trigger lists of the group: one copy
+        HIPCHK(hipMemcpyAsync(c->stage_h[slot], c->stage_d, c->stage_bytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipEventRecord(c->evA[slot], c->stream));
         return TDF_OK;
     };
@@ -1045,12 +1045,12 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
         HIPCHK(hipEventSynchronize(c->evA[slot]));
         for (int k = 0; k < gsize(gi); k++) {
             const int j = grp(gi, k);
-            S[j].aa = c->scal_hb[slot][4 * k];
-            S[j].bb = c->scal_hb[slot][4 * k + 1];
+            S[j].aa = stage_scal(c, c->stage_h[slot])[4 * k];
+            S[j].bb = stage_scal(c, c->stage_h[slot])[4 * k + 1];
             skies[j] = S[j];
             const int iss = S[j].iss_size;
             if (iss <= 0) continue;
-            const int *cnt = c->cnt_hb[slot] + (size_t)k * c->cnt_hb_cap;
+            const int *cnt = stage_cnt(c, c->stage_h[slot]) + (size_t)k * c->cnt_hb_cap;
             for (int t = 0; t < iss; t++)
                 if (cnt[1 + t] > maxtrig) overflow = true;
             bool again = cnt[0] > prefix;
@@ -1063,7 +1063,8 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
                 continue;
             }
             nrec_per[j] = iss;
-            int r = fill_records(c, S[j], mm[j], nn[j], o, iss, c->trig_hb[slot] + (size_t)k * BATCH_TRIG_PREFIX,
+            int r = fill_records(c, S[j], mm[j], nn[j], o, iss,
+                                 stage_trig(c, c->stage_h[slot]) + (size_t)k * BATCH_TRIG_PREFIX,
                                  cnt[0], recs + roff[j], pairs, pair_cap, &np);
             if (r) return r;
         }
@@ -1338,20 +1339,23 @@ int tdf_autotune(tdf_ctx *c)
                     sa.nslots = b;
                     sa.fr0 = 0;
                     sa.fr1 = n / 2 - 1;
+                    // one warm-up pair, then the mean of three pairs
                     float t = 0.f;
                     for (int rep = 0; rep < 2; rep++) {
                         HIPCHK(hipEventRecord(e0, c->stream));
-                        hipError_t er = d ? launch_spin_pass1<double>(p, sa, c->stream)
-                                          : launch_spin_pass1<float>(p, sa, c->stream);
-                        if (er == hipSuccess)
-                            er = d ? launch_spin_pass2<double>(p, sa, c->stream)
-                                   : launch_spin_pass2<float>(p, sa, c->stream);
-                        HIPCHK(er);
+                        for (int k = 0; k < (rep ? 3 : 1); k++) {
+                            hipError_t er = d ? launch_spin_pass1<double>(p, sa, c->stream)
+                                              : launch_spin_pass1<float>(p, sa, c->stream);
+                            if (er == hipSuccess)
+                                er = d ? launch_spin_pass2<double>(p, sa, c->stream)
+                                       : launch_spin_pass2<float>(p, sa, c->stream);
+                            HIPCHK(er);
+                        }
                         HIPCHK(hipEventRecord(e1, c->stream));
                         HIPCHK(hipEventSynchronize(e1));
                         HIPCHK(hipEventElapsedTime(&t, e0, e1));
                     }
-                    const float per = t / b;
+                    const float per = t / (3 * b);
                     if (per < best_t) {
                         best_t = per;
                         best = p;

@@ -260,7 +260,17 @@ __device__ __forceinline__ void stockham_stage(cx<T> *tile, const cx<T> *__restr
     constexpr int LR = L / RS;
     constexpr int LP = ldsgeom<L>::LP;
     const int tid = threadIdx.x;
-    cx<T> v[16];
+    cx<T> v[16], w[16];
+    // twiddles are loaded before the LDS reads and the barrier, so that their
+    // latency overlaps them
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int k = ((tid + u * NT) % LR) % NS;
+#pragma unroll
+            for (int r = 1; r < RS; r++) w[u * RS + r] = twL[r * k * (L / (NS * RS))];
+        }
+    }
 #pragma unroll
     for (int u = 0; u < NB; u++) {
         const int g = tid + u * NT;
@@ -277,9 +287,9 @@ __device__ __forceinline__ void stockham_stage(cx<T> *tile, const cx<T> *__restr
         if constexpr (NS > 1) {
 #pragma unroll
             for (int r = 1; r < RS; r++) {
-                cx<T> w = twL[r * k * (L / (NS * RS))];
-                if constexpr (SIGN > 0) w.y = -w.y;
-                v[u * RS + r] = cmul(v[u * RS + r], w);
+                cx<T> wr = w[u * RS + r];
+                if constexpr (SIGN > 0) wr.y = -wr.y;
+                v[u * RS + r] = cmul(v[u * RS + r], wr);
             }
         }
         dftR<RS, SIGN>(&v[u * RS]);
@@ -325,6 +335,13 @@ __device__ __forceinline__ void stockham_last_regs(const cx<T> *tile, const cx<T
     constexpr int LR = L / RS;
     constexpr int LP = ldsgeom<L>::LP;
     const int tid = threadIdx.x;
+    cx<T> w[16];
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+        const int k = ((tid + u * NT) % LR) % NS;
+#pragma unroll
+        for (int r = 1; r < RS; r++) w[u * RS + r] = twL[r * k * (L / (NS * RS))];
+    }
 #pragma unroll
     for (int u = 0; u < NB; u++) {
         const int g = tid + u * NT;
@@ -334,14 +351,11 @@ __device__ __forceinline__ void stockham_last_regs(const cx<T> *tile, const cx<T
     }
 #pragma unroll
     for (int u = 0; u < NB; u++) {
-        const int g = tid + u * NT;
-        const int j = g % LR;
-        const int k = j % NS;
 #pragma unroll
         for (int r = 1; r < RS; r++) {
-            cx<T> w = twL[r * k * (L / (NS * RS))];
-            if constexpr (SIGN > 0) w.y = -w.y;
-            v[u * RS + r] = cmul(v[u * RS + r], w);
+            cx<T> wr = w[u * RS + r];
+            if constexpr (SIGN > 0) wr.y = -wr.y;
+            v[u * RS + r] = cmul(v[u * RS + r], wr);
         }
         dftR<RS, SIGN>(&v[u * RS]);
     }
