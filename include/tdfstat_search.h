@@ -1,0 +1,107 @@
+/*
+ * tdfstat_search.h -- C host driver over libtdfstat_hip.so
+ * (libtdfstat_search.so): the sky loop of search() with the HDF5 trigger
+ * file and checkpoint/resume.
+ *
+ *   search/network/openmp/jobcore.c:28-188   search()
+ *   search/network/openmp/io.c:16-281         hdfout_init / hdfout_extend /
+ *                                              hdfout_finalize
+ *   search/network/openmp/init.c:761-809      read_checkpoints
+ *   search/network/openmp/init.c:510-705      set_search_range (range modes)
+ *   search/network/openmp/auxi.c:63-90,264-332  dimless_to_code, gridr
+ *
+ * The trigger file has the reference's layout (format_version 2): root
+ * attributes format_version, git_commit, t_start (t_end, totsgnl, walltime,
+ * num_threads at the end), compound attributes "opts", "sett", "s_range",
+ * "ifo" with the reference's member names and types, and the extendible
+ * dataset "triggers" of compound {m, n, s, ra, dec, fdot: float; ffstat:
+ * variable-length float (f, F pairs)}.
+ */
+#ifndef TDFSTAT_SEARCH_H
+#define TDFSTAT_SEARCH_H
+
+#include "tdfstat_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Search_range (struct.h:69-82) */
+typedef struct {
+    int pmr[2], fr[2];
+    float mr[2], nr[2], spndr[2];
+    float mstep, nstep, sstep;
+    float mst, nst, sst;
+    int pst;
+} tdf_range;
+
+/* What the trigger file records about the run: the Command_line_opts,
+ * Search_settings and Detector_settings members hdfout_init writes
+ * (io.c:36-130).  Strings may be NULL. */
+typedef struct {
+    /* Command_line_opts */
+    const char *indir, *outdir, *fstat_norm, *grid_file, *usedet, *addsig, *mods, *label;
+    const char *gtype, *gcenter, *gsizes, *gsteps;
+    int band, seg, hemi, veto_flag, checkp_flag;
+    double thr, overlap, narrowdown;
+    /* Search_settings */
+    double fpo, dt, B, oms, omr, Smin, Smax, sepsm, cepsm;
+    int nfft, nod, N, nfftf, nmax, nmin, s, nd, interpftpad, fftpad, Ninterp, nifo;
+    int numlines_band, nvlines_all_inband, bufsize, dd;
+    double M[16];
+    /* Detector_settings (name: 2 characters) */
+    const char *det_name[8], *det_xdatname[8];
+    double det_start_time[8];
+} tdf_run_info;
+
+/* hdfout_init (io.c:16-186): creates the file (truncating it) with the
+ * run's attributes and an empty "triggers" dataset. */
+int tdf_h5_init(const char *path, const tdf_run_info *run, const tdf_range *r);
+/* hdfout_extend (io.c:190-252): appends nrec records; the (f, F) pairs of
+ * record k are pairs[2*recs[k].offset .. 2*(recs[k].offset + recs[k].ntrig)). */
+int tdf_h5_extend(const char *path, const tdf_record *recs, long long nrec, const float *pairs);
+/* hdfout_finalize (io.c:256-281) */
+int tdf_h5_finalize(const char *path, long long totsgnl, double walltime, int nthreads);
+/* reading back (tests, tools): number of records and of (f, F) pairs, then
+ * the records (offsets into pairs as above) */
+int tdf_h5_count(const char *path, long long *nrec, long long *npairs);
+int tdf_h5_read(const char *path, tdf_record *recs, long long rec_cap, float *pairs, long long pair_cap);
+
+/* gridr (auxi.c:264-332): integer grid ranges of spindown, nn and mm that
+ * cover the band (|f| <= pi), spindowns in [-Smax, 0] and the sky
+ * (|al1|, |al2| <= oms + pi), from the grid matrix M. */
+void tdf_gridr(const double M[16], double oms, double Smax, float spndr[2], float nr[2], float mr[2]);
+/* dimless_to_code (auxi.c:63-90): grid coordinates x with M^T x = pos. */
+int tdf_dimless_to_code(const double M[16], const double pos_dimless[4], float pos_code[4]);
+/* set_search_range (init.c:510-705).  gtype 0: allsky (gridr, fr = [nmin,
+ * nmax]); 1: allsky_range (centre rounded to the integer grid); 2:
+ * directed_range (exact centre).  gcenter = f [Hz], fdot [Hz/s], ra, dec
+ * [rad]; gsizes = f bins (integer), s, mm, nn half-widths; gsteps = f
+ * (ignored, 1), s, mm, nn steps.  hemi 0: both hemispheres. */
+int tdf_set_search_range(const tdf_run_info *run, int gtype, int hemi, const float gcenter[4],
+                         const float gsizes[4], const float gsteps[4], tdf_range *r);
+
+/* read_checkpoints (init.c:761-809): with checkp_flag and a readable state
+ * file "pst mst nst sst FNum", resume there (an empty state file means the
+ * run had finished: TDF_ERR_STATE); otherwise start at the range minima. */
+int tdf_read_checkpoint(const char *state_file, int checkp_flag, tdf_range *r, long long *fnum);
+
+/* search() (jobcore.c:28-188): for each hemisphere pm = pst..pmr[1], the
+ * file <outdir>/triggers_<seg:3>_<band:4><label>_<pm>.h5, the mm/nn loops
+ * from (mst, nst) with job_core for each point (one tdf_job_core_batch call
+ * per mm row), the records appended to the file after each row and, with
+ * state_file non-NULL, the checkpoint "pm mm nn+1 sst FNum" written after
+ * each row (the reference writes it after each point) and emptied at the
+ * end.  On resume an existing trigger file is appended to (the reference
+ * truncates it, io.c:138).  *fnum counts records, *totsgnl (f, F) pairs.
+ * o's spindown and frequency ranges (sst, spndr1, sstep, fr0, fr1) are
+ * taken from r; its other fields (thr, lines, bufsize, ...) are used as
+ * given.  The state file name is the caller's: the reference uses
+ * state_<seg:3>_<band:4><label>[_<hemi>].dat (init.c:769-775). */
+int tdf_search(tdf_ctx *ctx, const tdf_run_info *run, tdf_range *r, const tdf_job_opts *o,
+               const char *state_file, long long *fnum, long long *totsgnl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -117,7 +117,72 @@ SIGNATURES = {
     "tdf_synchronize": (C.c_int, [_P]),
 }
 
+# libtdfstat_search.so (include/tdfstat_search.h): the C host driver
+SEARCH_LIB_PATH = os.path.join(_HERE, "libtdfstat_search.so")
+
+
+class TdfRange(C.Structure):
+    _fields_ = [
+        ("pmr", C.c_int * 2), ("fr", C.c_int * 2),
+        ("mr", C.c_float * 2), ("nr", C.c_float * 2), ("spndr", C.c_float * 2),
+        ("mstep", C.c_float), ("nstep", C.c_float), ("sstep", C.c_float),
+        ("mst", C.c_float), ("nst", C.c_float), ("sst", C.c_float),
+        ("pst", C.c_int),
+    ]
+
+
+class TdfRunInfo(C.Structure):
+    _fields_ = (
+        [(f, C.c_char_p) for f in ("indir", "outdir", "fstat_norm", "grid_file", "usedet", "addsig", "mods",
+                                   "label", "gtype", "gcenter", "gsizes", "gsteps")]
+        + [(f, C.c_int) for f in ("band", "seg", "hemi", "veto_flag", "checkp_flag")]
+        + [(f, C.c_double) for f in ("thr", "overlap", "narrowdown")]
+        + [(f, C.c_double) for f in ("fpo", "dt", "B", "oms", "omr", "Smin", "Smax", "sepsm", "cepsm")]
+        + [(f, C.c_int) for f in ("nfft", "nod", "N", "nfftf", "nmax", "nmin", "s", "nd", "interpftpad",
+                                  "fftpad", "Ninterp", "nifo", "numlines_band", "nvlines_all_inband",
+                                  "bufsize", "dd")]
+        + [("M", C.c_double * 16), ("det_name", C.c_char_p * 8), ("det_xdatname", C.c_char_p * 8),
+           ("det_start_time", C.c_double * 8)]
+    )
+
+
+_LLP = C.POINTER(C.c_longlong)
+SEARCH_SIGNATURES = {
+    "tdf_h5_init": (C.c_int, [C.c_char_p, C.POINTER(TdfRunInfo), C.POINTER(TdfRange)]),
+    "tdf_h5_extend": (C.c_int, [C.c_char_p, C.POINTER(TdfRecord), C.c_longlong, _FP]),
+    "tdf_h5_finalize": (C.c_int, [C.c_char_p, C.c_longlong, C.c_double, C.c_int]),
+    "tdf_h5_count": (C.c_int, [C.c_char_p, _LLP, _LLP]),
+    "tdf_h5_read": (C.c_int, [C.c_char_p, C.POINTER(TdfRecord), C.c_longlong, _FP, C.c_longlong]),
+    "tdf_gridr": (None, [_DP, C.c_double, C.c_double, _FP, _FP, _FP]),
+    "tdf_dimless_to_code": (C.c_int, [_DP, _DP, _FP]),
+    "tdf_set_search_range": (C.c_int, [C.POINTER(TdfRunInfo), C.c_int, C.c_int, _FP, _FP, _FP,
+                                       C.POINTER(TdfRange)]),
+    "tdf_read_checkpoint": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(TdfRange), _LLP]),
+    "tdf_search": (C.c_int, [_P, C.POINTER(TdfRunInfo), C.POINTER(TdfRange), C.POINTER(TdfJobOpts),
+                             C.c_char_p, _LLP, _LLP]),
+}
+
 _lib = None
+_search_lib = None
+
+
+def load_search():
+    """Load libtdfstat_search.so (which loads libtdfstat_hip.so); raise if it
+    has not been built."""
+    global _search_lib
+    if _search_lib is not None:
+        return _search_lib
+    load()
+    if not os.path.exists(SEARCH_LIB_PATH):
+        raise RuntimeError(
+            f"{SEARCH_LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(SEARCH_LIB_PATH)
+    for name, (res, args) in SEARCH_SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _search_lib = lib
+    return lib
 
 
 class TdfError(RuntimeError):

@@ -9,6 +9,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtdfstat_hip.so")
 SOURCES = ["tdf_kernels.hip", "tdf_capi.hip", "tdf_vetting.hip"]
+HOST_LIB = os.path.join(HERE, "libtdfstat_search.so")
+HOST_SOURCES = ["tdf_h5.c", "tdf_range.c", "tdf_search.c"]
+HDF5_PREFIX = os.environ.get("TDF_HDF5_PREFIX", "/opt/conda")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 are generated.
@@ -39,7 +42,24 @@ def build(verbose=False):
     if _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose):
         raise RuntimeError("hipcc failed linking libtdfstat_hip.so")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose=False):
+    """libtdfstat_search.so: the C host driver (search loop, HDF5 trigger
+    file, checkpoint, search ranges) over libtdfstat_hip.so; host code only,
+    built with gcc against the image's HDF5 (/opt/conda)."""
+    srcs = [os.path.join(CSRC, "host", s) for s in HOST_SOURCES]
+    tmp = HOST_LIB + ".tmp"
+    cmd = (["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-Wno-unused-result",
+            "-I", os.path.join(os.path.dirname(HERE), "include"), "-I", HDF5_PREFIX + "/include", "-o", tmp]
+           + srcs + ["-L", HERE, "-ltdfstat_hip", "-L", HDF5_PREFIX + "/lib", "-lhdf5_hl", "-lhdf5", "-lm",
+                     "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + HDF5_PREFIX + "/lib"])
+    if _run(cmd, verbose):
+        raise RuntimeError("gcc failed building libtdfstat_search.so")
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

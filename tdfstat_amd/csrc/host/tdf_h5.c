@@ -1,0 +1,363 @@
+/* Trigger file of the search (HDF5), with the reference's layout:
+ * hdfout_init / hdfout_extend / hdfout_finalize (io.c:16-281).  HDF5
+ * compound members are matched by name, so the in-memory structs below
+ * only need the reference's member names and types, not its struct layout. */
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <hdf5.h>
+#include <hdf5_hl.h>
+
+#include "tdfstat_search.h"
+
+#define TRIG_DSET_NAME "triggers"
+#define FORMAT_VERSION 2
+#define DETNAME_LENGTH 2
+#define FNAME_LENGTH 2048
+
+/* one record of the "triggers" dataset (struct.h:211-215) */
+typedef struct {
+    float m, n, s, ra, dec, fdot;
+    hvl_t ffstat;
+} h5_trigger;
+
+typedef struct {
+    const char *indir, *outdir;
+    int band, seg, hemi;
+    double thr, overlap, narrowdown;
+    const char *fstat_norm, *grid_file, *usedet, *addsig, *mods, *label;
+    int veto_flag, checkp_flag;
+    const char *gtype, *gcenter, *gsizes, *gsteps;
+} h5_opts;
+
+typedef struct {
+    double fpo, dt, B, oms, omr, Smin, Smax, sepsm, cepsm;
+    int nfft, nod, N, nfftf, nmax, nmin, s, nd, interpftpad, fftpad, Ninterp, nifo, numlines_band,
+        nvlines_all_inband, bufsize, dd;
+    double M[16];
+} h5_sett;
+
+typedef struct {
+    char name[DETNAME_LENGTH];
+    char xdatname[FNAME_LENGTH];
+    double start_time;
+} h5_ifo;
+
+static hid_t trigger_type(hid_t *ffstat_type)
+{
+    *ffstat_type = H5Tvlen_create(H5T_NATIVE_FLOAT);
+    hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(h5_trigger));
+    H5Tinsert(t, "m", HOFFSET(h5_trigger, m), H5T_NATIVE_FLOAT);
+    H5Tinsert(t, "n", HOFFSET(h5_trigger, n), H5T_NATIVE_FLOAT);
+    H5Tinsert(t, "s", HOFFSET(h5_trigger, s), H5T_NATIVE_FLOAT);
+    H5Tinsert(t, "ra", HOFFSET(h5_trigger, ra), H5T_NATIVE_FLOAT);
+    H5Tinsert(t, "dec", HOFFSET(h5_trigger, dec), H5T_NATIVE_FLOAT);
+    H5Tinsert(t, "fdot", HOFFSET(h5_trigger, fdot), H5T_NATIVE_FLOAT);
+    H5Tinsert(t, "ffstat", HOFFSET(h5_trigger, ffstat), *ffstat_type);
+    return t;
+}
+
+static const char *nz(const char *s) { return s ? s : ""; }
+
+static void stamp(char *buf, size_t n)
+{
+    time_t now = time(NULL);
+    struct tm *t = localtime(&now);
+    strftime(buf, n, "%Y-%m-%d %H:%M:%S", t);
+}
+
+int tdf_h5_init(const char *path, const tdf_run_info *run, const tdf_range *r)
+{
+    if (!path || !run || !r || run->nifo < 0 || run->nifo > 8) return TDF_ERR_ARG;
+    hid_t vstr = H5Tcopy(H5T_C_S1);
+    H5Tset_size(vstr, H5T_VARIABLE);
+    hid_t ffstat_t;
+    hid_t trig_t = trigger_type(&ffstat_t);
+
+    /* opts (io.c:45-71) */
+    hid_t opts_t = H5Tcreate(H5T_COMPOUND, sizeof(h5_opts));
+    H5Tinsert(opts_t, "indir", HOFFSET(h5_opts, indir), vstr);
+    H5Tinsert(opts_t, "outdir", HOFFSET(h5_opts, outdir), vstr);
+    H5Tinsert(opts_t, "band", HOFFSET(h5_opts, band), H5T_NATIVE_INT);
+    H5Tinsert(opts_t, "seg", HOFFSET(h5_opts, seg), H5T_NATIVE_INT);
+    H5Tinsert(opts_t, "hemi", HOFFSET(h5_opts, hemi), H5T_NATIVE_INT);
+    H5Tinsert(opts_t, "thr", HOFFSET(h5_opts, thr), H5T_NATIVE_DOUBLE);
+    H5Tinsert(opts_t, "overlap", HOFFSET(h5_opts, overlap), H5T_NATIVE_DOUBLE);
+    H5Tinsert(opts_t, "narrowdown", HOFFSET(h5_opts, narrowdown), H5T_NATIVE_DOUBLE);
+    H5Tinsert(opts_t, "fstat_norm", HOFFSET(h5_opts, fstat_norm), vstr);
+    H5Tinsert(opts_t, "grid_file", HOFFSET(h5_opts, grid_file), vstr);
+    H5Tinsert(opts_t, "usedet", HOFFSET(h5_opts, usedet), vstr);
+    H5Tinsert(opts_t, "addsig", HOFFSET(h5_opts, addsig), vstr);
+    H5Tinsert(opts_t, "mods", HOFFSET(h5_opts, mods), vstr);
+    H5Tinsert(opts_t, "label", HOFFSET(h5_opts, label), vstr);
+    H5Tinsert(opts_t, "veto_flag", HOFFSET(h5_opts, veto_flag), H5T_NATIVE_INT);
+    H5Tinsert(opts_t, "checkp_flag", HOFFSET(h5_opts, checkp_flag), H5T_NATIVE_INT);
+    H5Tinsert(opts_t, "gtype", HOFFSET(h5_opts, gtype), vstr);
+    H5Tinsert(opts_t, "gcenter", HOFFSET(h5_opts, gcenter), vstr);
+    H5Tinsert(opts_t, "gsizes", HOFFSET(h5_opts, gsizes), vstr);
+    H5Tinsert(opts_t, "gsteps", HOFFSET(h5_opts, gsteps), vstr);
+    h5_opts op = {nz(run->indir), nz(run->outdir), run->band, run->seg, run->hemi, run->thr, run->overlap,
+                  run->narrowdown, nz(run->fstat_norm), nz(run->grid_file), nz(run->usedet), nz(run->addsig),
+                  nz(run->mods), nz(run->label), run->veto_flag, run->checkp_flag, nz(run->gtype),
+                  nz(run->gcenter), nz(run->gsizes), nz(run->gsteps)};
+
+    /* sett (io.c:74-101) */
+    hid_t sett_t = H5Tcreate(H5T_COMPOUND, sizeof(h5_sett));
+#define SD(f) H5Tinsert(sett_t, #f, HOFFSET(h5_sett, f), H5T_NATIVE_DOUBLE)
+#define SI(f) H5Tinsert(sett_t, #f, HOFFSET(h5_sett, f), H5T_NATIVE_INT)
+    SD(fpo); SD(dt); SD(B); SD(oms); SD(omr); SD(Smin); SD(Smax); SD(sepsm); SD(cepsm);
+    SI(nfft); SI(nod); SI(N); SI(nfftf); SI(nmax); SI(nmin); SI(s); SI(nd); SI(interpftpad); SI(fftpad);
+    SI(Ninterp); SI(nifo); SI(numlines_band); SI(nvlines_all_inband); SI(bufsize); SI(dd);
+#undef SD
+#undef SI
+    hsize_t d16 = 16;
+    hid_t M_t = H5Tarray_create2(H5T_NATIVE_DOUBLE, 1, &d16);
+    H5Tinsert(sett_t, "M", HOFFSET(h5_sett, M), M_t);
+    h5_sett sp;
+    memset(&sp, 0, sizeof sp);
+    sp.fpo = run->fpo; sp.dt = run->dt; sp.B = run->B; sp.oms = run->oms; sp.omr = run->omr;
+    sp.Smin = run->Smin; sp.Smax = run->Smax; sp.sepsm = run->sepsm; sp.cepsm = run->cepsm;
+    sp.nfft = run->nfft; sp.nod = run->nod; sp.N = run->N; sp.nfftf = run->nfftf; sp.nmax = run->nmax;
+    sp.nmin = run->nmin; sp.s = run->s; sp.nd = run->nd; sp.interpftpad = run->interpftpad;
+    sp.fftpad = run->fftpad; sp.Ninterp = run->Ninterp; sp.nifo = run->nifo;
+    sp.numlines_band = run->numlines_band; sp.nvlines_all_inband = run->nvlines_all_inband;
+    sp.bufsize = run->bufsize; sp.dd = run->dd;
+    memcpy(sp.M, run->M, sizeof sp.M);
+
+    /* s_range (io.c:110-125) */
+    hid_t range_t = H5Tcreate(H5T_COMPOUND, sizeof(tdf_range));
+    hsize_t d2 = 2;
+    hid_t i2 = H5Tarray_create2(H5T_NATIVE_INT, 1, &d2);
+    hid_t f2 = H5Tarray_create2(H5T_NATIVE_FLOAT, 1, &d2);
+    H5Tinsert(range_t, "pmr", HOFFSET(tdf_range, pmr), i2);
+    H5Tinsert(range_t, "fr", HOFFSET(tdf_range, fr), i2);
+    H5Tinsert(range_t, "mr", HOFFSET(tdf_range, mr), f2);
+    H5Tinsert(range_t, "nr", HOFFSET(tdf_range, nr), f2);
+    H5Tinsert(range_t, "spndr", HOFFSET(tdf_range, spndr), f2);
+    H5Tinsert(range_t, "mstep", HOFFSET(tdf_range, mstep), H5T_NATIVE_FLOAT);
+    H5Tinsert(range_t, "nstep", HOFFSET(tdf_range, nstep), H5T_NATIVE_FLOAT);
+    H5Tinsert(range_t, "sstep", HOFFSET(tdf_range, sstep), H5T_NATIVE_FLOAT);
+    H5Tinsert(range_t, "mst", HOFFSET(tdf_range, mst), H5T_NATIVE_FLOAT);
+    H5Tinsert(range_t, "nst", HOFFSET(tdf_range, nst), H5T_NATIVE_FLOAT);
+    H5Tinsert(range_t, "sst", HOFFSET(tdf_range, sst), H5T_NATIVE_FLOAT);
+    H5Tinsert(range_t, "pst", HOFFSET(tdf_range, pst), H5T_NATIVE_INT);
+
+    /* ifo (io.c:128-135) */
+    hid_t ifo_t = H5Tcreate(H5T_COMPOUND, sizeof(h5_ifo));
+    hid_t name_t = H5Tcopy(H5T_C_S1);
+    H5Tset_size(name_t, DETNAME_LENGTH);
+    hid_t xdat_t = H5Tcopy(H5T_C_S1);
+    H5Tset_size(xdat_t, FNAME_LENGTH);
+    H5Tinsert(ifo_t, "name", HOFFSET(h5_ifo, name), name_t);
+    H5Tinsert(ifo_t, "xdatname", HOFFSET(h5_ifo, xdatname), xdat_t);
+    H5Tinsert(ifo_t, "start_time", HOFFSET(h5_ifo, start_time), H5T_NATIVE_DOUBLE);
+    h5_ifo *ifo = (h5_ifo *)calloc(run->nifo > 0 ? run->nifo : 1, sizeof(h5_ifo));
+    for (int d = 0; d < run->nifo; d++) {
+        if (run->det_name[d]) memcpy(ifo[d].name, run->det_name[d], strnlen(run->det_name[d], DETNAME_LENGTH));
+        if (run->det_xdatname[d])
+            strncpy(ifo[d].xdatname, run->det_xdatname[d], FNAME_LENGTH - 1);
+        ifo[d].start_time = run->det_start_time[d];
+    }
+
+    int rc = TDF_OK;
+    hid_t file = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (file < 0) {
+        rc = TDF_ERR_ARG;
+    } else {
+        int fv = FORMAT_VERSION;
+        char ts[80];
+        stamp(ts, sizeof ts);
+        H5LTset_attribute_int(file, "/", "format_version", &fv, 1);
+        H5LTset_attribute_string(file, "/", "git_commit", "tdfstat_amd");
+        H5LTset_attribute_string(file, "/", "t_start", ts);
+        hid_t scalar = H5Screate(H5S_SCALAR);
+        hid_t a = H5Acreate2(file, "opts", opts_t, scalar, H5P_DEFAULT, H5P_DEFAULT);
+        if (a < 0 || H5Awrite(a, opts_t, &op) < 0) rc = TDF_ERR_ARG;
+        H5Aclose(a);
+        a = H5Acreate2(file, "sett", sett_t, scalar, H5P_DEFAULT, H5P_DEFAULT);
+        if (a < 0 || H5Awrite(a, sett_t, &sp) < 0) rc = TDF_ERR_ARG;
+        H5Aclose(a);
+        a = H5Acreate2(file, "s_range", range_t, scalar, H5P_DEFAULT, H5P_DEFAULT);
+        if (a < 0 || H5Awrite(a, range_t, r) < 0) rc = TDF_ERR_ARG;
+        H5Aclose(a);
+        hsize_t nifo = (hsize_t)run->nifo;
+        hid_t ifo_sp = H5Screate_simple(1, &nifo, NULL);
+        a = H5Acreate2(file, "ifo", ifo_t, ifo_sp, H5P_DEFAULT, H5P_DEFAULT);
+        if (a < 0 || H5Awrite(a, ifo_t, ifo) < 0) rc = TDF_ERR_ARG;
+        H5Aclose(a);
+        H5Sclose(ifo_sp);
+        H5Sclose(scalar);
+        /* empty extendible dataset, chunks of 3 records (io.c:172-178) */
+        hsize_t dim = 0, maxdim = H5S_UNLIMITED, chunk = 3;
+        hid_t prop = H5Pcreate(H5P_DATASET_CREATE);
+        H5Pset_chunk(prop, 1, &chunk);
+        hid_t space = H5Screate_simple(1, &dim, &maxdim);
+        hid_t ds = H5Dcreate2(file, TRIG_DSET_NAME, trig_t, space, H5P_DEFAULT, prop, H5P_DEFAULT);
+        if (ds < 0) rc = TDF_ERR_ARG;
+        H5Dclose(ds);
+        H5Sclose(space);
+        H5Pclose(prop);
+        H5Fclose(file);
+    }
+    free(ifo);
+    H5Tclose(name_t);
+    H5Tclose(xdat_t);
+    H5Tclose(ifo_t);
+    H5Tclose(i2);
+    H5Tclose(f2);
+    H5Tclose(range_t);
+    H5Tclose(M_t);
+    H5Tclose(sett_t);
+    H5Tclose(opts_t);
+    H5Tclose(trig_t);
+    H5Tclose(ffstat_t);
+    H5Tclose(vstr);
+    return rc;
+}
+
+int tdf_h5_extend(const char *path, const tdf_record *recs, long long nrec, const float *pairs)
+{
+    if (!path || nrec < 0 || (nrec > 0 && !recs)) return TDF_ERR_ARG;
+    if (nrec == 0) return TDF_OK;
+    h5_trigger *buf = (h5_trigger *)malloc(sizeof(h5_trigger) * (size_t)nrec);
+    if (!buf) return TDF_ERR_NOMEM;
+    for (long long k = 0; k < nrec; k++) {
+        const tdf_record *r = &recs[k];
+        buf[k].m = r->m;
+        buf[k].n = r->n;
+        buf[k].s = r->s;
+        buf[k].ra = r->ra;
+        buf[k].dec = r->dec;
+        buf[k].fdot = r->fdot;
+        buf[k].ffstat.len = 2 * (size_t)r->ntrig;
+        buf[k].ffstat.p = r->ntrig > 0 ? (void *)(pairs + 2 * r->offset) : NULL;
+    }
+    hid_t ffstat_t;
+    hid_t trig_t = trigger_type(&ffstat_t);
+    int rc = TDF_ERR_ARG;
+    hid_t file = H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT);
+    if (file >= 0) {
+        hid_t ds = H5Dopen2(file, TRIG_DSET_NAME, H5P_DEFAULT);
+        if (ds >= 0) {
+            hid_t fs = H5Dget_space(ds);
+            hsize_t dim = 0;
+            H5Sget_simple_extent_dims(fs, &dim, NULL);
+            H5Sclose(fs);
+            hsize_t start = dim, count = (hsize_t)nrec;
+            dim += count;
+            if (H5Dset_extent(ds, &dim) >= 0) {
+                fs = H5Dget_space(ds);
+                H5Sselect_hyperslab(fs, H5S_SELECT_SET, &start, NULL, &count, NULL);
+                hid_t ms = H5Screate_simple(1, &count, NULL);
+                if (H5Dwrite(ds, trig_t, ms, fs, H5P_DEFAULT, buf) >= 0) rc = TDF_OK;
+                H5Sclose(ms);
+                H5Sclose(fs);
+            }
+            H5Dclose(ds);
+        }
+        H5Fclose(file);
+    }
+    H5Tclose(trig_t);
+    H5Tclose(ffstat_t);
+    free(buf);
+    return rc;
+}
+
+int tdf_h5_finalize(const char *path, long long totsgnl, double walltime, int nthreads)
+{
+    if (!path) return TDF_ERR_ARG;
+    hid_t file = H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT);
+    if (file < 0) return TDF_ERR_ARG;
+    char ts[80];
+    stamp(ts, sizeof ts);
+    int tot = (int)totsgnl;  /* the reference's attribute is an int (io.c:272) */
+    H5LTset_attribute_string(file, "/", "t_end", ts);
+    H5LTset_attribute_int(file, "/", "totsgnl", &tot, 1);
+    H5LTset_attribute_double(file, "/", "walltime", &walltime, 1);
+    H5LTset_attribute_int(file, "/", "num_threads", &nthreads, 1);
+    H5Fclose(file);
+    return TDF_OK;
+}
+
+/* reads the whole dataset; the caller frees *out with read_free */
+static int read_all(const char *path, h5_trigger **out, hsize_t *n, hid_t *trig_t, hid_t *ffstat_t, hid_t *space)
+{
+    *trig_t = trigger_type(ffstat_t);
+    *out = NULL;
+    *space = -1;
+    hid_t file = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (file < 0) return TDF_ERR_ARG;
+    int rc = TDF_ERR_ARG;
+    hid_t ds = H5Dopen2(file, TRIG_DSET_NAME, H5P_DEFAULT);
+    if (ds >= 0) {
+        *space = H5Dget_space(ds);
+        H5Sget_simple_extent_dims(*space, n, NULL);
+        *out = (h5_trigger *)calloc(*n > 0 ? *n : 1, sizeof(h5_trigger));
+        if (*out && (*n == 0 || H5Dread(ds, *trig_t, H5S_ALL, H5S_ALL, H5P_DEFAULT, *out) >= 0)) rc = TDF_OK;
+        H5Dclose(ds);
+    }
+    H5Fclose(file);
+    return rc;
+}
+
+static void read_free(h5_trigger *buf, hsize_t n, hid_t trig_t, hid_t ffstat_t, hid_t space)
+{
+    if (buf && n > 0 && space >= 0) H5Dvlen_reclaim(trig_t, space, H5P_DEFAULT, buf);
+    free(buf);
+    if (space >= 0) H5Sclose(space);
+    H5Tclose(trig_t);
+    H5Tclose(ffstat_t);
+}
+
+int tdf_h5_count(const char *path, long long *nrec, long long *npairs)
+{
+    if (!path || !nrec || !npairs) return TDF_ERR_ARG;
+    h5_trigger *buf;
+    hsize_t n = 0;
+    hid_t tt, ft, sp;
+    int rc = read_all(path, &buf, &n, &tt, &ft, &sp);
+    if (rc == TDF_OK) {
+        long long np = 0;
+        for (hsize_t k = 0; k < n; k++) np += (long long)buf[k].ffstat.len / 2;
+        *nrec = (long long)n;
+        *npairs = np;
+    }
+    read_free(buf, n, tt, ft, sp);
+    return rc;
+}
+
+int tdf_h5_read(const char *path, tdf_record *recs, long long rec_cap, float *pairs, long long pair_cap)
+{
+    if (!path || (rec_cap > 0 && !recs) || (pair_cap > 0 && !pairs)) return TDF_ERR_ARG;
+    h5_trigger *buf;
+    hsize_t n = 0;
+    hid_t tt, ft, sp;
+    int rc = read_all(path, &buf, &n, &tt, &ft, &sp);
+    if (rc == TDF_OK) {
+        long long np = 0;
+        if ((long long)n > rec_cap) rc = TDF_ERR_CAPACITY;
+        for (hsize_t k = 0; k < n && rc == TDF_OK; k++) {
+            const long long m = (long long)buf[k].ffstat.len / 2;
+            if (np + m > pair_cap) {
+                rc = TDF_ERR_CAPACITY;
+                break;
+            }
+            tdf_record *r = &recs[k];
+            r->m = buf[k].m;
+            r->n = buf[k].n;
+            r->s = buf[k].s;
+            r->ra = buf[k].ra;
+            r->dec = buf[k].dec;
+            r->fdot = buf[k].fdot;
+            r->ntrig = (int)m;
+            r->offset = np;
+            if (m > 0) memcpy(pairs + 2 * np, buf[k].ffstat.p, sizeof(float) * 2 * (size_t)m);
+            np += m;
+        }
+    }
+    read_free(buf, n, tt, ft, sp);
+    return rc;
+}

@@ -1,0 +1,161 @@
+/* The sky loop of search() (jobcore.c:28-188) over libtdfstat_hip.so: one
+ * trigger file per hemisphere, job_core for every sky point (one pipelined
+ * tdf_job_core_batch call per mm row), records appended after each row,
+ * checkpoint state after each row. */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "tdfstat_search.h"
+
+static double now_s(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_REALTIME, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+/* state file "pm mm nn sst FNum" (jobcore.c:145-147), rewritten in place */
+static int write_state(FILE *st, int pm, float mm, float nn, float sst, long long fnum)
+{
+    if (!st) return TDF_OK;
+    if (ftruncate(fileno(st), 0) != 0) return TDF_ERR_ARG;
+    fseek(st, 0, SEEK_SET);
+    fprintf(st, "%d %f %f %f %d\n", pm, mm, nn, sst, (int)fnum);
+    fflush(st);
+    fseek(st, 0, SEEK_SET);
+    return TDF_OK;
+}
+
+/* job_core for the points (pm, mm, nn[0..n)) with growing pair capacity;
+ * the records and pairs stay in *recs / *pairs */
+static int run_row(tdf_ctx *ctx, int pm, float mm, const float *nn, int n, const tdf_job_opts *o, int max_sd,
+                   tdf_record **recs, long long *rec_cap, float **pairs, long long *pair_cap, long long *nrec,
+                   long long *npairs)
+{
+    int *pmv = (int *)malloc(sizeof(int) * n);
+    float *mmv = (float *)malloc(sizeof(float) * n);
+    tdf_sky *skies = (tdf_sky *)malloc(sizeof(tdf_sky) * n);
+    int *nper = (int *)malloc(sizeof(int) * n);
+    int rc = (pmv && mmv && skies && nper) ? TDF_OK : TDF_ERR_NOMEM;
+    for (int k = 0; k < n && rc == TDF_OK; k++) {
+        pmv[k] = pm;
+        mmv[k] = mm;
+    }
+    const long long need = (long long)n * max_sd;
+    if (rc == TDF_OK && need > *rec_cap) {
+        free(*recs);
+        *recs = (tdf_record *)malloc(sizeof(tdf_record) * need);
+        *rec_cap = *recs ? need : 0;
+        if (!*recs) rc = TDF_ERR_NOMEM;
+    }
+    while (rc == TDF_OK) {
+        rc = tdf_job_core_batch(ctx, n, pmv, mmv, nn, o, skies, nper, *recs, *rec_cap, *pairs, *pair_cap, nrec,
+                                npairs);
+        if (rc != TDF_ERR_CAPACITY) break;
+        /* more (f, F) pairs than the buffer holds: grow it and redo the row */
+        const long long cap = *pair_cap > 0 ? 4 * *pair_cap : 1 << 20;
+        free(*pairs);
+        *pairs = (float *)malloc(sizeof(float) * 2 * cap);
+        *pair_cap = *pairs ? cap : 0;
+        rc = *pairs ? TDF_OK : TDF_ERR_NOMEM;
+    }
+    free(pmv);
+    free(mmv);
+    free(skies);
+    free(nper);
+    return rc;
+}
+
+int tdf_search(tdf_ctx *ctx, const tdf_run_info *run, tdf_range *r, const tdf_job_opts *o_in,
+               const char *state_file, long long *fnum, long long *totsgnl)
+{
+    if (!ctx || !run || !r || !o_in || !fnum || !totsgnl || !(r->mstep > 0) || !(r->nstep > 0)) return TDF_ERR_ARG;
+    /* the spindown and frequency ranges come from r, as job_core reads them
+     * from s_range (jobcore.c:395-412, 482-514) */
+    tdf_job_opts o = *o_in;
+    o.spndr1 = r->spndr[1];
+    o.sstep = r->sstep;
+    o.fr0 = r->fr[0];
+    o.fr1 = r->fr[1];
+    const int max_sd = (int)((r->spndr[1] - r->spndr[0]) / r->sstep + 1);
+    FILE *st = NULL;
+    int rc = TDF_OK;
+    if (state_file) {
+        st = fopen(state_file, "w");
+        if (!st) return TDF_ERR_ARG;
+        write_state(st, r->pst, r->mst, r->nst, r->sst, *fnum);
+    }
+    tdf_record *recs = NULL;
+    float *pairs = NULL;
+    long long rec_cap = 0, pair_cap = 0;
+    *totsgnl = 0;
+    /* resuming inside the first hemisphere of this call */
+    const int resumed = state_file && *fnum > 0;
+    const int pm0 = r->pst;
+    for (int pm = r->pst; pm <= r->pmr[1] && rc == TDF_OK; ++pm) {
+        const double t0 = now_s();
+        char outname[4096];
+        snprintf(outname, sizeof outname, "%s/triggers_%03d_%04d%s_%d.h5", run->outdir ? run->outdir : ".",
+                 run->seg, run->band, run->label ? run->label : "", pm);
+        /* a resumed run keeps the records already written in its file */
+        if (!(resumed && pm == pm0 && access(outname, F_OK) == 0)) {
+            remove(outname);
+            rc = tdf_h5_init(outname, run, r);
+        }
+        long long tot_pm = 0;
+        const int imm_size = (int)((r->mr[1] - r->mst) / r->mstep + 1);
+        for (int imm = 0; imm < imm_size && rc == TDF_OK; ++imm) {
+            const float mm = r->mst + imm * r->mstep;
+            const int inn_size = (int)((r->nr[1] - r->nst) / r->nstep + 1);
+            if (inn_size <= 0) {
+                r->nst = r->nr[0];
+                continue;
+            }
+            float *nn = (float *)malloc(sizeof(float) * inn_size);
+            if (!nn) {
+                rc = TDF_ERR_NOMEM;
+                break;
+            }
+            for (int inn = 0; inn < inn_size; ++inn) nn[inn] = r->nst + inn * r->nstep;
+            long long nrec = 0, npairs = 0;
+            int first = 0;
+            /* the first point after a resume starts at the saved spindown
+             * (jobcore.c:402-403 reads s_range->sst; search() then resets it) */
+            if (r->sst != r->spndr[0]) {
+                o.sst = r->sst;
+                rc = run_row(ctx, pm, mm, nn, 1, &o, max_sd, &recs, &rec_cap, &pairs, &pair_cap, &nrec, &npairs);
+                if (rc == TDF_OK) rc = tdf_h5_extend(outname, recs, nrec, pairs);
+                *fnum += nrec;
+                tot_pm += npairs;
+                r->sst = r->spndr[0];
+                first = 1;
+            }
+            o.sst = r->spndr[0];
+            if (rc == TDF_OK && inn_size > first) {
+                rc = run_row(ctx, pm, mm, nn + first, inn_size - first, &o, max_sd, &recs, &rec_cap, &pairs,
+                             &pair_cap, &nrec, &npairs);
+                if (rc == TDF_OK) rc = tdf_h5_extend(outname, recs, nrec, pairs);
+                *fnum += nrec;
+                tot_pm += npairs;
+            }
+            if (rc == TDF_OK) write_state(st, pm, mm, nn[inn_size - 1] + 1, r->sst, *fnum);
+            free(nn);
+            r->nst = r->nr[0];
+        }
+        r->mst = r->mr[0];
+        *totsgnl += tot_pm;
+        if (rc == TDF_OK) rc = tdf_h5_finalize(outname, tot_pm, now_s() - t0, 1);
+    }
+    if (st) {
+        /* an empty state file prevents a restart after a successful end */
+        if (rc == TDF_OK && ftruncate(fileno(st), 0) != 0) rc = TDF_ERR_ARG;
+        fclose(st);
+    }
+    free(recs);
+    free(pairs);
+    return rc;
+}

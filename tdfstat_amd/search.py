@@ -300,6 +300,81 @@ def _records(recs, n, pairs):
     return out
 
 
+def run_info(sett, *, band=0, seg=1, hemi=0, thr=20.0, overlap=0.03125, narrowdown=0.5, label="",
+             outdir=".", indir=".", dets=(), det_names=(), checkp_flag=0, gtype=""):
+    """tdf_run_info for the trigger-file attributes (io.c:36-130).  The
+    returned structure keeps the byte strings it points to alive."""
+    ri = L.TdfRunInfo()
+    keep = []
+
+    def bs(v):
+        b = str(v).encode()
+        keep.append(b)
+        return b
+    ri.indir, ri.outdir, ri.label, ri.gtype = bs(indir), bs(outdir), bs(label), bs(gtype)
+    ri.fstat_norm = bs("")
+    ri.band, ri.seg, ri.hemi, ri.checkp_flag = band, seg, hemi, checkp_flag
+    ri.thr, ri.overlap, ri.narrowdown = thr, overlap, narrowdown
+    for f in ("fpo", "dt", "B", "oms", "omr", "Smin", "Smax", "sepsm", "cepsm"):
+        setattr(ri, f, float(getattr(sett, f)))
+    for f in ("nfft", "nod", "N", "nfftf", "nmax", "nmin", "s", "nd", "interpftpad", "fftpad", "Ninterp", "nifo",
+              "numlines_band", "nvlines_all_inband", "bufsize", "dd"):
+        setattr(ri, f, int(getattr(sett, f)))
+    for k in range(16):
+        ri.M[k] = float(sett.M[k])
+    names = list(det_names) or [getattr(d, "name", "") for d in dets]
+    for k, nm in enumerate(names[:8]):
+        ri.det_name[k] = bs(nm)
+        ri.det_xdatname[k] = bs(f"{indir}/xdatc_{seg:03d}_{band:04d}.bin")
+    ri._keep = keep
+    return ri
+
+
+def set_search_range(ri, gtype=0, hemi=0, gcenter=None, gsizes=None, gsteps=None):
+    """set_search_range (init.c:510-705) through libtdfstat_search.so.
+    gtype 0 allsky, 1 allsky_range, 2 directed_range."""
+    lib = L.load_search()
+    r = L.TdfRange()
+    arr = [None if v is None else np.ascontiguousarray(v, dtype=np.float32) for v in (gcenter, gsizes, gsteps)]
+    ptr = [L.fptr(a) if a is not None else C.POINTER(C.c_float)() for a in arr]
+    L.check(lib.tdf_set_search_range(C.byref(ri), gtype, hemi, *ptr, C.byref(r)), "tdf_set_search_range")
+    return r
+
+
+def read_checkpoint(r, state_file=None, checkp_flag=0):
+    """read_checkpoints (init.c:761-809): sets r's start point, returns FNum."""
+    lib = L.load_search()
+    fnum = C.c_longlong(0)
+    L.check(lib.tdf_read_checkpoint(state_file.encode() if state_file else None, checkp_flag, C.byref(r),
+                                    C.byref(fnum)), "tdf_read_checkpoint")
+    return fnum.value
+
+
+def h5_read(path):
+    """Records of a trigger file written by tdf_search / tdf_h5_extend."""
+    lib = L.load_search()
+    nrec, npairs = C.c_longlong(0), C.c_longlong(0)
+    L.check(lib.tdf_h5_count(path.encode(), C.byref(nrec), C.byref(npairs)), "tdf_h5_count")
+    recs = (L.TdfRecord * max(1, nrec.value))()
+    pairs = np.zeros(2 * max(1, npairs.value), dtype=np.float32)
+    L.check(lib.tdf_h5_read(path.encode(), recs, nrec.value, L.fptr(pairs), npairs.value), "tdf_h5_read")
+    return _records(recs, nrec.value, pairs)
+
+
+def search_file(ctx, ri, r, opts, state_file=None, fnum=0):
+    """search() (jobcore.c:28-188) through libtdfstat_search.so: writes
+    <outdir>/triggers_<seg>_<band><label>_<pm>.h5 per hemisphere and, with
+    state_file, the checkpoint.  r (TdfRange, with its start point set) is
+    updated in place.  Returns (FNum, totsgnl)."""
+    lib = L.load_search()
+    o = opts.to_c()
+    fn = C.c_longlong(fnum)
+    tot = C.c_longlong(0)
+    L.check(lib.tdf_search(ctx.h, C.byref(ri), C.byref(r), C.byref(o),
+                           state_file.encode() if state_file else None, C.byref(fn), C.byref(tot)), "tdf_search")
+    return fn.value, tot.value
+
+
 def search(ctx, sky_points, opts):
     """Sky loop of search() (jobcore.c:88-158): sky_points is an iterable of
     (pm, mm, nn).  Returns (records, FNum)."""

@@ -1,5 +1,6 @@
-"""The C-ABI library loads on a machine without a GPU and exports every
-function declared in include/tdfstat_hip.h (no compute calls here)."""
+"""The C-ABI libraries load on a machine without a GPU and export every
+function declared in include/tdfstat_hip.h and include/tdfstat_search.h (no
+compute calls here)."""
 import ctypes
 import os
 import re
@@ -9,8 +10,8 @@ from tdfstat_amd import _lib as L
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    txt = open(os.path.join(ROOT, "include", "tdfstat_hip.h")).read()
+def _declared(header="tdfstat_hip.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(tdf_[a-z0-9_]+)\s*\(", txt)))
 
@@ -31,3 +32,11 @@ def test_strerror_without_gpu():
     lib = L.load()
     assert lib.tdf_strerror(L.TDF_OK) == b"ok"
     assert b"Too many triggers" in lib.tdf_strerror(-4)
+
+
+def test_search_library_exports_every_symbol():
+    names = _declared("tdfstat_search.h")
+    assert set(names) == set(L.SEARCH_SIGNATURES), set(names) ^ set(L.SEARCH_SIGNATURES)
+    lib = L.load_search()
+    for name in names:
+        assert hasattr(lib, name), name

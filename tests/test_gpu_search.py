@@ -1,0 +1,135 @@
+"""GPU: the C host driver (tdf_search in libtdfstat_search.so) writes one
+HDF5 trigger file per hemisphere whose records are those of job_core at
+every grid point in loop order, and a run resumed from its checkpoint
+state ends with the same files as an uninterrupted one."""
+import os
+
+import numpy as np
+import pytest
+
+from tdfstat_amd import _lib as L
+from tdfstat_amd import search as TS
+from tdfstat_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _segment():
+    rng = np.random.default_rng(41)
+    return synth.make_segment(nod=2, det_names=("H1", "L1"), seed=41,
+                              injections=[synth.random_injection(rng, 14.0) for _ in range(3)])
+
+
+def _range(sett, mr=(-1, 1), nr=(-2, 1), spndr=(-3, 3), pmr=(1, 2)):
+    r = L.TdfRange()
+    r.pmr[:] = list(pmr)
+    r.fr[:] = [sett.nmin, sett.nmax]
+    r.mr[:] = list(mr)
+    r.nr[:] = list(nr)
+    r.spndr[:] = list(spndr)
+    r.mstep = r.nstep = r.sstep = 1.0
+    return r
+
+
+def _same(got, want):
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert (g.m, g.n, g.s, g.ra, g.dec, g.fdot) == (w.m, w.n, w.s, w.ra, w.dec, w.fdot)
+        np.testing.assert_array_equal(g.ffstat, w.ffstat)
+
+
+def _fname(d, pm, label=""):
+    return os.path.join(str(d), f"triggers_001_0372{label}_{pm}.h5")
+
+
+@pytest.mark.parametrize("prec", [L.TDF_PREC_FP32, L.TDF_PREC_FP64])
+def test_search_file_equals_job_core(tmp_path, prec):
+    _flush(f"search file prec={prec}")
+    seg = _segment()
+    sett = seg.sett
+    opts = TS.default_opts(sett, thr=6.0)
+    r = _range(sett)
+    ri = TS.run_info(sett, band=372, seg=1, dets=seg.dets, outdir=str(tmp_path), thr=6.0)
+    assert TS.read_checkpoint(r) == 0
+    with TS.Context(sett, seg.dets, prec=prec) as ctx:
+        fnum, tot = TS.search_file(ctx, ri, r, opts)
+        # job_core point by point, in the order of the reference's loops
+        want = {1: [], 2: []}
+        o = TS.default_opts(sett, thr=6.0, sst=-3, spndr1=3, sstep=1)
+        for pm in (1, 2):
+            for mm in (-1.0, 0.0, 1.0):
+                for nn in (-2.0, -1.0, 0.0, 1.0):
+                    _sky, recs = ctx.job_core(pm, mm, nn, o, pair_cap=1 << 20)
+                    want[pm].extend(recs)
+    for pm in (1, 2):
+        _same(TS.h5_read(_fname(tmp_path, pm)), want[pm])
+    assert fnum == len(want[1]) + len(want[2])
+    assert tot == sum(len(w.ffstat) for pm in (1, 2) for w in want[pm])
+    assert tot > 0
+    _flush(f"  {fnum} records, {tot} triggers")
+
+
+def test_checkpoint_resume_gives_the_same_files(tmp_path):
+    _flush("checkpoint resume")
+    seg = _segment()
+    sett = seg.sett
+    opts = TS.default_opts(sett, thr=6.0)
+    full, part = tmp_path / "full", tmp_path / "part"
+    full.mkdir()
+    part.mkdir()
+    with TS.Context(sett, seg.dets) as ctx:
+        # uninterrupted run, with checkpointing on
+        st_full = str(full / "state.dat")
+        r = _range(sett)
+        ri = TS.run_info(sett, band=372, seg=1, dets=seg.dets, outdir=str(full), checkp_flag=1)
+        TS.read_checkpoint(r, st_full, 1)
+        fnum_full, _ = TS.search_file(ctx, ri, r, opts, state_file=st_full)
+        assert open(st_full).read() == ""  # emptied at the end
+        with pytest.raises(L.TdfError):  # and a restart then has nothing to do
+            TS.read_checkpoint(_range(sett), st_full, 1)
+        # an interrupted run: hemisphere 1, rows mm = -1, 0 done, then the
+        # state the driver writes after the row mm = 0
+        ri_p = TS.run_info(sett, band=372, seg=1, dets=seg.dets, outdir=str(part), checkp_flag=1)
+        r = _range(sett, mr=(-1, 0), pmr=(1, 1))
+        TS.read_checkpoint(r)
+        fnum_done, _ = TS.search_file(ctx, ri_p, r, opts)
+        st = str(part / "state.dat")
+        with open(st, "w") as f:
+            f.write(f"1 {0.0:f} {2.0:f} {-3.0:f} {fnum_done}\n")
+        r = _range(sett)
+        assert TS.read_checkpoint(r, st, 1) == fnum_done
+        assert (r.pst, r.mst, r.nst, r.sst) == (1, 0, 2, -3)
+        fnum, _ = TS.search_file(ctx, ri_p, r, opts, state_file=st, fnum=fnum_done)
+    assert fnum == fnum_full
+    for pm in (1, 2):
+        _same(TS.h5_read(_fname(part, pm)), TS.h5_read(_fname(full, pm)))
+
+
+def test_resume_inside_a_point_starts_at_the_saved_spindown(tmp_path):
+    """A state line with sst above spndr[0] resumes its first point at that
+    spindown (job_core reads s_range->sst, jobcore.c:395-412) and the later
+    points at spndr[0]."""
+    _flush("resume at spindown")
+    seg = _segment()
+    sett = seg.sett
+    opts = TS.default_opts(sett, thr=6.0)
+    st = str(tmp_path / "state.dat")
+    with open(st, "w") as f:
+        f.write(f"2 {1.0:f} {0.0:f} {0.0:f} 0\n")
+    r = _range(sett)
+    ri = TS.run_info(sett, band=372, seg=1, dets=seg.dets, outdir=str(tmp_path), checkp_flag=1)
+    TS.read_checkpoint(r, st, 1)
+    with TS.Context(sett, seg.dets) as ctx:
+        TS.search_file(ctx, ri, r, opts, state_file=st)
+        want = []
+        for nn, sst in ((0.0, 0.0), (1.0, -3.0)):
+            o = TS.default_opts(sett, thr=6.0, sst=sst, spndr1=3, sstep=1)
+            want.extend(ctx.job_core(2, 1.0, nn, o, pair_cap=1 << 20)[1])
+    got = TS.h5_read(_fname(tmp_path, 2))
+    _same(got, want)
+    assert not os.path.exists(_fname(tmp_path, 1))
+    assert min(g.s for g in got if g.n == 0.0) >= 0

@@ -1,0 +1,224 @@
+"""CPU: the C host driver library (libtdfstat_search.so) without a GPU --
+search ranges (gridr, dimless_to_code, set_search_range), the checkpoint
+reader and the HDF5 trigger file (layout and round trip)."""
+import ctypes as C
+import json
+import math
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from tdfstat_amd import _lib as L
+from tdfstat_amd import search as TS
+from tdfstat_amd import settings as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KAT = json.load(open(os.path.join(ROOT, "tests", "golden", "docs_known_answers.json")))
+DOCS_M = np.array(KAT["grid_matrix"]["M"], dtype=np.float64)
+DOCS_SETT = KAT["search_settings"]["output"]
+GRID_2DAY = np.array(KAT["grid_2day_dt2"]["M"], dtype=np.float64)
+H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+
+
+def _gridr(M, oms, Smax):
+    lib = L.load_search()
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    sp, nr, mr = (np.zeros(2, np.float32) for _ in range(3))
+    lib.tdf_gridr(L.dptr(M), oms, Smax, L.fptr(sp), L.fptr(nr), L.fptr(mr))
+    return sp, nr, mr
+
+
+def test_gridr_docs_kat():
+    """docs/search_for_candidates.md:330-343: the example grid's allsky
+    search range is mr [-10, 10], nr [-5, 5], spndr [-153, 63]."""
+    sp, nr, mr = _gridr(DOCS_M, DOCS_SETT["oms"], DOCS_SETT["Smax"])
+    want = KAT["gridr_ranges"]
+    assert list(mr) == want["mr"] and list(nr) == want["nr"] and list(sp) == want["spndr"]
+
+
+@pytest.mark.parametrize("M", [DOCS_M, GRID_2DAY], ids=["docs", "2day"])
+def test_gridr_covers_the_box_corners(M):
+    """Every corner of [-pi, pi] x [-Smax, 0] x [-cof, cof]^2 maps (x = M^-T b,
+    numpy's LAPACK solve) inside the returned integer ranges, and each range
+    end is attained by a corner (floor/ceil of an extreme)."""
+    oms, Smax = 81.4851, 6.70414e-10 if M is DOCS_M else 2e-9
+    sp, nr, mr = _gridr(M, oms, Smax)
+    cof = oms + math.pi
+    xs = []
+    for f in (math.pi, -math.pi):
+        for s in (-Smax, 0.0):
+            for a in (cof, -cof):
+                for b in (cof, -cof):
+                    xs.append(np.linalg.solve(M.reshape(4, 4).T, [f, s, a, b]))
+    xs = np.array(xs)
+    for col, rng in ((1, sp), (2, nr), (3, mr)):
+        assert rng[0] == np.floor(xs[:, col].min()) and rng[1] == np.ceil(xs[:, col].max())
+
+
+def test_dimless_to_code_inverts_the_grid():
+    """M^T x = pos: a code-unit grid point maps back to itself."""
+    lib = L.load_search()
+    M = np.ascontiguousarray(GRID_2DAY)
+    for code in ([3.0, -7.0, 12.0, -4.0], [0.0, 1.0, -1.0, 2.0]):
+        pos = M.reshape(4, 4).T @ np.array(code)
+        out = np.zeros(4, np.float32)
+        L.check(lib.tdf_dimless_to_code(L.dptr(M), L.dptr(np.ascontiguousarray(pos)), L.fptr(out)), "d2c")
+        np.testing.assert_allclose(out, code, atol=1e-4)
+    sing = np.zeros(16)
+    out = np.zeros(4, np.float32)
+    assert lib.tdf_dimless_to_code(L.dptr(sing), L.dptr(np.zeros(4)), L.fptr(out)) == -1
+
+
+def _docs_sett():
+    st = S.SearchSettings(fpo=DOCS_SETT["fpo"], dt=0.5, nod=6)
+    for k, v in DOCS_SETT.items():
+        if k != "bufsize_allsky" and hasattr(st, k):
+            setattr(st, k, v)
+    st.B = 0.5 / (2 * st.dt)
+    st.M = DOCS_M.copy()
+    st.nifo = 2
+    return st
+
+
+def test_set_search_range_allsky():
+    """gtype allsky (init.c:560-566): gridr ranges, fr = [nmin, nmax], both
+    hemispheres unless one is asked for."""
+    ri = TS.run_info(_docs_sett(), band=17, det_names=("H1", "L1"))
+    r = TS.set_search_range(ri, 0, 0)
+    want = KAT["gridr_ranges"]
+    assert list(r.pmr) == want["pmr"] and list(r.fr) == want["fr"]
+    assert list(r.mr) == want["mr"] and list(r.nr) == want["nr"] and list(r.spndr) == want["spndr"]
+    assert (r.mstep, r.nstep, r.sstep) == (1, 1, 1)
+    r = TS.set_search_range(ri, 0, 2)
+    assert list(r.pmr) == [2, 2]
+
+
+def _ecl_lin(alpha, delta, sepsm, cepsm):
+    """Linear sky coordinates of (alpha, delta): the equatorial unit vector
+    rotated to the ecliptic frame, components (y_ecl, x_ecl)."""
+    v = np.array([math.cos(delta) * math.cos(alpha), math.cos(delta) * math.sin(alpha), math.sin(delta)])
+    y_ecl = cepsm * v[1] + sepsm * v[2]
+    return y_ecl, v[0], (-sepsm * v[1] + cepsm * v[2])
+
+
+@pytest.mark.parametrize("alpha,delta", [(1.1, 0.4), (4.0, -0.7)])
+def test_set_search_range_directed(alpha, delta):
+    """gtype directed/allsky range (init.c:571-660): the centre is the grid
+    point of (fdot, alpha, delta); the hemisphere is the sign of the ecliptic
+    z component; allsky_range rounds the centre, directed keeps it."""
+    st = _docs_sett()
+    ri = TS.run_info(st, band=17)
+    fdot, f0 = -3e-10, st.fpo + 0.3
+    steps = np.array([1, 0.5, 2, 1], np.float32)
+    zero = np.zeros(4, np.float32)
+    r2 = TS.set_search_range(ri, 2, 0, [f0, fdot, alpha, delta], zero, steps)
+    y, x, z = _ecl_lin(alpha, delta, st.sepsm, st.cepsm)
+    assert r2.pmr[0] == r2.pmr[1] == (1 if z >= 0 else 2)
+    # the centre solves M^T c = (0, pi fdot dt^2, oms y, oms x) in code units
+    pos = np.array([0.0, math.pi * fdot * st.dt ** 2, st.oms * y, st.oms * x])
+    c = np.linalg.solve(DOCS_M.reshape(4, 4).T, pos)
+    assert r2.spndr[0] == r2.spndr[1] and r2.nr[0] == r2.nr[1] and r2.mr[0] == r2.mr[1]
+    np.testing.assert_allclose([r2.spndr[0], r2.nr[0], r2.mr[0]], c[1:], rtol=1e-5, atol=1e-3)
+    assert (r2.sstep, r2.mstep, r2.nstep) == (0.5, 2, 1)
+    sizes = np.array([5, 2.2, 3.9, 1.2], np.float32)
+    r1 = TS.set_search_range(ri, 1, 0, [f0, fdot, alpha, delta], sizes, steps)
+    # half-widths rounded to whole steps, centres rounded to integers
+    assert r1.spndr[0] == round(c[1]) - 2.0 and r1.spndr[1] == round(c[1]) + 2.0
+    assert r1.mr[0] == round(c[3]) - 4.0 and r1.mr[1] == round(c[3]) + 4.0
+    assert r1.nr[0] == round(c[2]) - 1.0 and r1.nr[1] == round(c[2]) + 1.0
+    assert r1.fr[1] - r1.fr[0] == 10
+
+
+def test_read_checkpoint(tmp_path):
+    """read_checkpoints (init.c:761-809): no state -> range minima; a state
+    line -> resume point and FNum; an empty state file -> finished run."""
+    r = L.TdfRange()
+    r.pmr[:] = [1, 2]
+    r.mr[:] = [-3, 3]
+    r.nr[:] = [-2, 2]
+    r.spndr[:] = [-10, 4]
+    st = tmp_path / "state.dat"
+    assert TS.read_checkpoint(r, str(st), 1) == 0
+    assert (r.pst, r.mst, r.nst, r.sst) == (1, -3, -2, -10)
+    st.write_text("2 1.000000 -1.000000 -7.000000 42\n")
+    assert TS.read_checkpoint(r, str(st), 0) == 0  # checkpointing off: ignored
+    assert (r.pst, r.mst) == (1, -3)
+    assert TS.read_checkpoint(r, str(st), 1) == 42
+    assert (r.pst, r.mst, r.nst, r.sst) == (2, 1, -1, -7)
+    st.write_text("")
+    with pytest.raises(L.TdfError):
+        TS.read_checkpoint(r, str(st), 1)
+
+
+def _records(rng, n):
+    recs = (L.TdfRecord * n)()
+    pairs, off = [], 0
+    for k in range(n):
+        nt = int(rng.integers(0, 4))
+        recs[k].m, recs[k].n, recs[k].s = float(k), float(-k), float(rng.integers(-50, 5))
+        recs[k].ra, recs[k].dec, recs[k].fdot = rng.random(), rng.random() - 0.5, -1e-10 * rng.random()
+        recs[k].ntrig, recs[k].offset = nt, off
+        pairs.append(rng.random((nt, 2)).astype(np.float32))
+        off += nt
+    return recs, np.ascontiguousarray(np.concatenate(pairs).reshape(-1)) if off else np.zeros(2, np.float32)
+
+
+def test_h5_round_trip(tmp_path):
+    """hdfout_init / extend / finalize: records appended in two calls read
+    back in order with their (f, F) pairs, zero-trigger records included."""
+    lib = L.load_search()
+    st = _docs_sett()
+    ri = TS.run_info(st, band=17, seg=3, det_names=("H1", "L1"), label="_t")
+    r = TS.set_search_range(ri, 0, 0)
+    path = str(tmp_path / "triggers_003_0017_t_1.h5").encode()
+    L.check(lib.tdf_h5_init(path, C.byref(ri), C.byref(r)), "init")
+    rng = np.random.default_rng(5)
+    all_recs = []
+    for n in (7, 4):
+        recs, pairs = _records(rng, n)
+        L.check(lib.tdf_h5_extend(path, recs, n, L.fptr(pairs)), "extend")
+        all_recs.extend(TS._records(recs, n, pairs))
+    L.check(lib.tdf_h5_finalize(path, 123, 4.5, 1), "finalize")
+    got = TS.h5_read(path.decode())
+    assert len(got) == len(all_recs) == 11
+    for g, w in zip(got, all_recs):
+        assert (g.m, g.n, g.s, g.ra, g.dec, g.fdot) == (w.m, w.n, w.s, w.ra, w.dec, w.fdot)
+        np.testing.assert_array_equal(g.ffstat, w.ffstat)
+    assert any(len(w.ffstat) == 0 for w in all_recs)
+    # an empty dataset reads as no records
+    p2 = str(tmp_path / "empty.h5").encode()
+    L.check(lib.tdf_h5_init(p2, C.byref(ri), C.byref(r)), "init")
+    assert TS.h5_read(p2.decode()) == []
+
+
+@pytest.mark.skipif(H5DUMP is None, reason="h5dump not installed")
+def test_h5_layout_is_the_reference_schema(tmp_path):
+    """The file carries what io.c:16-281 writes and coincidences/io.c:175-330
+    reads: root attributes, compound attributes with the reference's member
+    names and types, and the "triggers" dataset of {m, n, s, ra, dec, fdot,
+    ffstat: vlen float}."""
+    lib = L.load_search()
+    ri = TS.run_info(_docs_sett(), band=17, seg=3, det_names=("H1", "L1"))
+    r = TS.set_search_range(ri, 0, 0)
+    path = str(tmp_path / "t.h5")
+    L.check(lib.tdf_h5_init(path.encode(), C.byref(ri), C.byref(r)), "init")
+    recs, pairs = _records(np.random.default_rng(1), 3)
+    L.check(lib.tdf_h5_extend(path.encode(), recs, 3, L.fptr(pairs)), "extend")
+    L.check(lib.tdf_h5_finalize(path.encode(), 9, 1.0, 1), "finalize")
+    hdr = subprocess.run([H5DUMP, "-H", path], capture_output=True, text=True, check=True).stdout
+    for attr in ("format_version", "git_commit", "t_start", "t_end", "totsgnl", "walltime", "num_threads",
+                 "opts", "sett", "s_range", "ifo"):
+        assert f'ATTRIBUTE "{attr}"' in hdr, attr
+    for member in ("band", "seg", "hemi", "overlap", "narrowdown", "grid_file", "fpo", "dt", "B", "nod", "N",
+                   "nfftf", "numlines_band", "nvlines_all_inband", "pmr", "spndr", "pst", "start_time",
+                   "xdatname"):
+        assert f'"{member}";' in hdr, member
+    assert 'DATASET "triggers"' in hdr
+    assert re.search(r'H5T_VLEN \{\s*H5T_IEEE_F32LE\s*\}\s*"ffstat";', hdr)
+    full = subprocess.run([H5DUMP, "-a", "/format_version", path], capture_output=True, text=True,
+                          check=True).stdout
+    assert "(0): 2" in full
