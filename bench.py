@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILED_STEPS = 40
 
 # Workloads of the reference configurations (realised odd x odd grids).
 CONFIGS = {
@@ -126,8 +127,6 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
     for k in range(warmup):
         step(k)
     ctx.synchronize()
-    if profile:
-        ctx.profile(True, spindown_only=True)
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -140,8 +139,19 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
     ctx.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    kt = ctx.kernel_times() if profile else {}
-    return ctx, opts, nt, el, kt, ntrig
+    kt, ntp = {}, 0
+    if profile:
+        # kernel durations: HIP events around every spindown kernel on the
+        # library stream, over a repetition of the same steps right after the
+        # timed ones (an event between two kernels delays the second by
+        # ~10 us, so they stay out of the throughput run)
+        ctx.profile(True, spindown_only=True)
+        for k in range(min(steps, PROFILED_STEPS)):
+            ntp += step(warmup + k)[0]
+        ctx.synchronize()
+        kt = ctx.kernel_times()
+        ctx.profile(False)
+    return ctx, opts, nt, el, kt, ntrig, ntp
 
 
 def fp32_vs_fp64_error(cfg, seg, pt, opts, ctx32, ctx64, nss=8):
@@ -249,7 +259,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="max spindowns per launch (0 = library default)")
     ap.add_argument("--no-tune", action="store_true", help="use the default plan instead of tdf_autotune")
     ap.add_argument("--plan", default=None,
-                    help="FP32 plan n1,e1,e2,fused,batch instead of tdf_autotune (profiling runs)")
+                    help="FP32 plan n1,e1,e2,fused,batch[,ntz] instead of tdf_autotune (profiling runs)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
     ap.add_argument("--per-step", type=int, default=0,
                     help="sky points per step (one tdf_job_core_batch call); 1 = one tdf_job_core call per step; "
@@ -281,17 +291,17 @@ def main():
           f"sky={len(pts)} S={S}", file=sys.stderr, flush=True)
 
     plan = tuple(int(v) for v in args.plan.split(",")) if args.plan else None
-    if plan is not None and len(plan) != 5:
-        ap.error("--plan takes n1,e1,e2,fused,batch")
+    if plan is not None and len(plan) not in (5, 6):
+        ap.error("--plan takes n1,e1,e2,fused,batch[,ntz]")
     per_step = args.per_step or cfg["per_step"]
-    ctx32, opts, nt, el, kt, ntrig = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
+    ctx32, opts, nt, el, kt, ntrig, ntp = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
                                              args.batch, not args.no_tune, plan, per_step)
     value = nt / el
     print(f"[bench] fp32: {nt} templates in {el:.3f} s -> {value:.0f} templates/s", file=sys.stderr, flush=True)
     fp64 = fperr = None
     if not args.no_fp64:
         k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 8)
-        ctx64, _, nt64, el64, kt64, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
+        ctx64, _, nt64, el64, kt64, _, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
                                                 not args.no_tune, None, per_step)
         value64 = nt64 / el64
         print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
@@ -305,7 +315,7 @@ def main():
     spin = {k: v for k, v in kt.items() if k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat", "triggers")}
     dom = max(spin, key=lambda k: spin[k][0])
     ms_tot, launches = spin[dom]
-    nslots_avg = nt / max(1, kt["spin_pass1_demod_fft"][1])
+    nslots_avg = ntp / max(1, kt["spin_pass1_demod_fft"][1])
     line_plan = ctx32.plan_info()
     kb = kernel_bytes(sett, S, nslots_avg, dom, False)
     ach = kb / (ms_tot / launches * 1e-3) / 1e9
@@ -339,6 +349,7 @@ def main():
         "fp64_path": fp64,
         "fp32_vs_fp64": fperr,
         "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},
+        "kernels_ms_templates": ntp,
         "cpu_baseline": None,
     }
     if not args.no_cpu:

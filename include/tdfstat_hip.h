@@ -248,8 +248,10 @@ int tdf_mfma_probe(const float *A, const float *B, float *D);
  * on timing). */
 int tdf_autotune(tdf_ctx *ctx);
 /* the plan in use: pass-2 length n1 (nfftf = n1 * n2), LDS tile elements
- * e1/e2 of the two passes, fused = pass 2 reads Z and writes F directly from
- * registers, batch = spindowns per launch */
+ * e1/e2 of the two passes, fused = bit 0: pass 2 reads Z and writes F
+ * directly from registers, bit 1: the four-step intermediate Z is written
+ * and read with non-temporal accesses (set when a launch's Z goes to HBM,
+ * clear when it fits the memory-side cache), batch = spindowns per launch */
 int tdf_plan_info(tdf_ctx *ctx, int *n1, int *e1, int *e2, int *fused, int *batch);
 /* selects a plan explicitly (same meaning; batch <= the creation batch) */
 int tdf_set_plan(tdf_ctx *ctx, int n1, int e1, int e2, int fused, int batch);

@@ -245,6 +245,7 @@ int make_plan(tdf_ctx *c, int n, FftPlan *p, int n1_req = 0, int e1 = 0, int e2 
     p->l2 = ilog2i(n2);
     p->e1 = e1 > 0 ? e1 : emax;
     p->e2 = e2 > 0 ? e2 : emax;
+    p->ntz = 1;
     // LDS tiles: a power of two of 512..emax elements holding at least two columns
     if (!is_pow2(p->e1) || !is_pow2(p->e2) || p->e1 < 512 || p->e2 < 512 || p->e1 < 2 * n2 ||
         p->e2 < 2 * n1 || p->e1 > emax || p->e2 > emax)
@@ -1299,13 +1300,33 @@ int tdf_fft_c2c(const double *in, double *out, int n, int howmany, int sign, int
     return TDF_OK;
 }
 
+// Pass 1 reads the rows i >= N of xm and tmp1 and relies on their being
+// zero; the sky stage writes only the rows i < N, at positions that depend
+// on n1 (SpinLayout).  A change of n1 therefore clears every copy.
+static int clear_spin_inputs(tdf_ctx *c)
+{
+    const tdf_geom &g = c->g;
+    const size_t xs = c->csz * 2 * (size_t)g.nifo * g.nfftf, ts = sizeof(double) * (size_t)g.nifo * g.nfftf;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemset(c->xm, 0, xs));
+    HIPCHK(hipMemset(c->tmp1, 0, ts));
+    if (c->batch_ready) {
+        HIPCHK(hipMemset(c->xm_g, 0, (size_t)c->gmax * xs));
+        HIPCHK(hipMemset(c->tmp1_g, 0, (size_t)c->gmax * ts));
+    }
+    return TDF_OK;
+}
+
 int tdf_autotune(tdf_ctx *c)
 {
     if (!c) return TDF_ERR_ARG;
     const int n = c->g.nfftf;
     const bool d = c->prec == TDF_PREC_FP64;
     const int emax = d ? 2048 : 4096;
-    std::vector<int> n1s = {256, 512, 1024, 2048}, es = {512, 1024, 2048, 4096}, bs = {32, 64, 128, 256};
+    std::vector<int> n1s = {256, 512, 1024, 2048}, es = {512, 1024, 2048, 4096}, bs = {8, 16, 32, 64, 128, 256};
+    // Z of a launch that fits the memory-side cache is read back from it:
+    // ordinary Z accesses are tried for launches up to this size
+    const size_t zcache = 160ull << 20;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -1319,14 +1340,17 @@ int tdf_autotune(tdf_ctx *c)
     for (int n1 : n1s)
         for (int ea : es)
             for (int eb : es)
-                for (int f2 = 0; f2 < 2; f2++) {
+                for (int f2 = 0; f2 < 2; f2++)
+                for (int ntz = 0; ntz < 2; ntz++) {
                 if (ea > emax || eb > emax) continue;
                 FftPlan p{};
                 if (make_plan(c, n, &p, n1, ea, eb) != TDF_OK) continue;
                 if (f2 && (eb != 2 * n1 || n1 < 512 || (d && n1 > 1024))) continue;
                 p.f2 = f2;
+                p.ntz = ntz;
                 for (int b : bs) {
                     if (b > c->batch_max) continue;
+                    if (!ntz && (size_t)b * 2 * n * c->csz > zcache) continue;
                     SpinArgs sa{};
                     sa.xm = c->xm;
                     sa.tmp1 = c->tmp1;
@@ -1366,7 +1390,11 @@ int tdf_autotune(tdf_ctx *c)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->prof = prof;
-    if (best.n1 != c->p_spin.n1) c->sky_valid = false;  // per-sky inputs are laid out for n1
+    if (best.n1 != c->p_spin.n1) {
+        c->sky_valid = false;  // per-sky inputs are laid out for n1
+        int rc = clear_spin_inputs(c);
+        if (rc) return rc;
+    }
     c->p_spin = best;
     c->batch = best_b;
     c->tuned = true;
@@ -1379,7 +1407,7 @@ int tdf_plan_info(tdf_ctx *c, int *n1, int *e1, int *e2, int *fused, int *batch)
     if (n1) *n1 = c->p_spin.n1;
     if (e1) *e1 = c->p_spin.e1;
     if (e2) *e2 = c->p_spin.e2;
-    if (fused) *fused = c->p_spin.f2;
+    if (fused) *fused = c->p_spin.f2 | (c->p_spin.ntz << 1);
     if (batch) *batch = c->batch;
     return TDF_OK;
 }
@@ -1391,10 +1419,14 @@ int tdf_set_plan(tdf_ctx *c, int n1, int e1, int e2, int fused, int batch)
     FftPlan p{};
     int rc = make_plan(c, c->g.nfftf, &p, n1, e1, e2);
     if (rc) return rc;
-    if (fused && (p.e2 != 2 * p.n1 || p.n1 < 512 || (d && p.n1 > 1024))) return TDF_ERR_UNSUPPORTED;
-    p.f2 = fused ? 1 : 0;
+    if ((fused & 1) && (p.e2 != 2 * p.n1 || p.n1 < 512 || (d && p.n1 > 1024))) return TDF_ERR_UNSUPPORTED;
+    p.f2 = fused & 1;
+    p.ntz = (fused >> 1) & 1;
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (p.n1 != c->p_spin.n1) c->sky_valid = false;  // per-sky inputs are laid out for n1
+    if (p.n1 != c->p_spin.n1) {
+        c->sky_valid = false;  // per-sky inputs are laid out for n1
+        if ((rc = clear_spin_inputs(c))) return rc;
+    }
     c->p_spin = p;
     c->batch = batch;
     return TDF_OK;

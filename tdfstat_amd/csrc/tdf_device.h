@@ -57,6 +57,36 @@ __device__ __forceinline__ void ld_stream2(const cx<T> *p, cx<T> &a, cx<T> &b)
     a = cx<T>{v.x, v.y};
     b = cx<T>{v.z, v.w};
 }
+// Z (the four-step intermediate) accesses: non-temporal (NT) when a launch's
+// Z exceeds the memory-side cache and goes to HBM, ordinary when it fits
+// there and the next kernel reads it back from that cache.
+template <bool NT, typename T>
+__device__ __forceinline__ void st_z2(cx<T> *p, cx<T> a, cx<T> b)
+{
+    if constexpr (NT) {
+        st_stream2(p, a, b);
+    } else {
+        const vec4_t<T> v = {a.x, a.y, b.x, b.y};
+        *(vec4_t<T> *)p = v;
+    }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void ld_z2(const cx<T> *p, cx<T> &a, cx<T> &b)
+{
+    if constexpr (NT) {
+        ld_stream2(p, a, b);
+    } else {
+        const vec4_t<T> v = *(const vec4_t<T> *)p;
+        a = cx<T>{v.x, v.y};
+        b = cx<T>{v.z, v.w};
+    }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ cx<T> ld_z(const cx<T> *p)
+{
+    if constexpr (NT) return ld_stream(p);
+    else return *p;
+}
 template <typename T>
 __device__ __forceinline__ T cnorm(cx<T> a) { return a.x * a.x + a.y * a.y; }
 

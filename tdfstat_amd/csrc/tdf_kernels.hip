@@ -259,7 +259,7 @@ struct IO_P2T {
 // times W_nfftf^(c*k2) into Z[slot][k2*n1 + c][a|b] (Fa and Fb of one bin
 // interleaved: a workgroup's TC columns then form runs of TC*2 complex
 // values, 128 bytes for TC = 8 in FP32).
-template <typename T, int NIFO, bool HALF_>
+template <typename T, int NIFO, bool HALF_, bool NTZ>
 struct IO_SpinP1 {
     static constexpr bool HALF = HALF_;  // rows >= L/2 of every column are zero (N <= nfftf/2)
     const cx<T> *xm;       // [nifo][2][N]
@@ -280,40 +280,97 @@ struct IO_SpinP1 {
         const double hr = td.het1 * 0.15915494309189533577, sr = td.spnd * 0.15915494309189533577;
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         const int nd = NIFO > 0 ? NIFO : nifo;
-        // inputs are column-major (column c is contiguous over q): lanes
-        // along q read consecutive addresses
+        // Inputs are column-major (column c is contiguous over q): lanes along
+        // q read consecutive addresses.  When a wavefront's 64 lanes share one
+        // column and hold 64 consecutive rows (WU), "some row of this
+        // wavefront is below N" is uniform and tested on lane 0's row.  Rows
+        // i >= N need no per-lane test: xm and tmp1 are zero there (never
+        // written), so their demodulated products vanish.
         constexpr int LQ = HALF ? L / 2 : L;
+        constexpr int NIT = TC * LQ / NT;
+        constexpr bool WU = LQ % 64 == 0 && NT % 64 == 0;
+        static_assert(TC * LQ % NT == 0, "tile");
+        const cx<T> *xmb = xm + (size_t)c0 * L;
+        const double *tmb = tmp1 + (size_t)c0 * L;
+        // FP32 bulk: the phase is formed directly in turns (het1/2pi,
+        // spnd/2pi), reduced in FP64, sin/cos in hardware; FP64 path: the
+        // phase in radians as the reference forms it
+        const double hph = sizeof(T) == 4 ? hr : td.het1;
+        auto rot = [&](double ph0, double t1, cx<T> ma, cx<T> mb, cx<T> &pa, cx<T> &pb) {
+            T s, c;
+            if constexpr (sizeof(T) == 4) sincos_rev(fma(sr, t1, ph0), &s, &c);
+            else sincos_fp64arg(ph0 + td.spnd * t1, &s, &c);
+            pa = cx<T>{ma.x * c + ma.y * s, ma.y * c - ma.x * s};
+            pb = cx<T>{mb.x * c + mb.y * s, mb.y * c - mb.x * s};
+        };
+        if constexpr (NIFO > 0) {
+            // the loads of KB iterations are issued together (all in bounds:
+            // c < n1, q < L), then their products are formed
+            constexpr int KB0 = 8 / NIFO >= 8 ? 8 : 8 / NIFO >= 4 ? 4 : 8 / NIFO >= 2 ? 2 : 1;
+            constexpr int KB = KB0 < NIT ? KB0 : NIT;
+            static_assert(NIT % KB == 0, "batch");
 #pragma unroll
-        for (int e = threadIdx.x; e < TC * LQ; e += NT) {
-            const int q = e % LQ, cl = e / LQ;
-            const int i = c0 + cl + n1 * q;
-            const size_t t = (size_t)(c0 + cl) * L + q;
-            cx<T> fa = {(T)0, (T)0}, fb = {(T)0, (T)0};
-            if (i < N) {
-                // FP32 bulk: the phase is formed directly in turns (het1/2pi,
-                // spnd/2pi), reduced in FP64, sin/cos in hardware; FP64 path:
-                // the phase in radians as the reference forms it
-                const double ph0 = (sizeof(T) == 4 ? hr : td.het1) * (double)i;
+            for (int k0 = 0; k0 < NIT; k0 += KB) {
+                double t1v[KB][NIFO];
+                cx<T> mav[KB][NIFO], mbv[KB][NIFO];
 #pragma unroll
-                for (int d = 0; d < (NIFO > 0 ? NIFO : MAXDET); d++) {
-                    if (NIFO == 0 && d >= nd) break;
-                    T s, c;
-                    if constexpr (sizeof(T) == 4) {
-                        sincos_rev(fma(sr, tmp1[(size_t)d * n + t], ph0), &s, &c);
-                    } else {
-                        const double ph = ph0 + td.spnd * tmp1[(size_t)d * n + t];
-                        sincos_fp64arg(ph, &s, &c);
+                for (int u = 0; u < KB; u++) {
+                    const int e = threadIdx.x + (k0 + u) * NT;
+                    const int off = (e / LQ) * L + e % LQ;
+#pragma unroll
+                    for (int d = 0; d < NIFO; d++) {
+                        t1v[u][d] = tmb[(size_t)d * n + off];
+                        mav[u][d] = xmb[(size_t)(2 * d) * n + off];
+                        mbv[u][d] = xmb[(size_t)(2 * d + 1) * n + off];
                     }
-                    const cx<T> ma = xm[(size_t)(2 * d) * n + t];
-                    const cx<T> mb = xm[(size_t)(2 * d + 1) * n + t];
-                    fa.x += ma.x * c + ma.y * s;
-                    fa.y += ma.y * c - ma.x * s;
-                    fb.x += mb.x * c + mb.y * s;
-                    fb.y += mb.y * c - mb.x * s;
+                }
+#pragma unroll
+                for (int u = 0; u < KB; u++) {
+                    const int e = threadIdx.x + (k0 + u) * NT;
+                    const int q = e % LQ, cl = e / LQ;
+                    const int i = c0 + cl + n1 * q;
+                    cx<T> fa = {(T)0, (T)0}, fb = {(T)0, (T)0};
+                    if ((WU ? __builtin_amdgcn_readfirstlane(i) : i) < N) {
+                        const double ph0 = hph * (double)i;
+                        rot(ph0, t1v[u][0], mav[u][0], mbv[u][0], fa, fb);
+#pragma unroll
+                        for (int d = 1; d < NIFO; d++) {
+                            cx<T> pa, pb;
+                            rot(ph0, t1v[u][d], mav[u][d], mbv[u][d], pa, pb);
+                            fa = fa + pa;
+                            fb = fb + pb;
+                        }
+                    }
+                    tile[cl * LP + ldsgeom<L>::pad(q)] = fa;
+                    tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = fb;
                 }
             }
-            tile[cl * LP + ldsgeom<L>::pad(q)] = fa;
-            tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = fb;
+        } else {
+#pragma unroll
+            for (int k = 0; k < NIT; k++) {
+                const int e = threadIdx.x + k * NT;
+                const int q = e % LQ, cl = e / LQ;
+                const int off = cl * L + q;
+                const int i = c0 + cl + n1 * q;
+                cx<T> fa = {(T)0, (T)0}, fb = {(T)0, (T)0};
+                if ((WU ? __builtin_amdgcn_readfirstlane(i) : i) < N) {
+                    const double ph0 = hph * (double)i;
+                    for (int d = 0; d < nd; d++) {
+                        cx<T> pa, pb;
+                        rot(ph0, tmb[(size_t)d * n + off], xmb[(size_t)(2 * d) * n + off],
+                            xmb[(size_t)(2 * d + 1) * n + off], pa, pb);
+                        if (d == 0) {
+                            fa = pa;
+                            fb = pb;
+                        } else {
+                            fa = fa + pa;
+                            fb = fb + pb;
+                        }
+                    }
+                }
+                tile[cl * LP + ldsgeom<L>::pad(q)] = fa;
+                tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = fb;
+            }
         }
     }
     template <int SIGN, int L, int NCOL>
@@ -331,7 +388,7 @@ struct IO_SpinP1 {
         const cx<T> ws = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * (NT / TC)));
 #pragma unroll
         for (int k2 = k20; k2 < L; k2 += NT / TC) {
-            st_stream2(&Z[base + ((size_t)k2 * n1 + c) * 2], cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w),
+            st_z2<NTZ>(&Z[base + ((size_t)k2 * n1 + c) * 2], cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w),
                        cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w));
             w = cmul(w, ws);
         }
@@ -342,7 +399,7 @@ struct IO_SpinP1 {
 // FT[slot][k2*n1 + k1] (bin k = k2 + n2*k1).  Without FStat normalisation
 // every bin of the MAX_ALG-3 candidate range with F >= thr is also appended
 // to a candidate list (wavefront ballot, one atomic per wave).
-template <typename T>
+template <typename T, bool NTZ>
 struct IO_SpinP2 {
     static constexpr bool HALF = false;
     const cx<T> *Z;
@@ -364,7 +421,7 @@ struct IO_SpinP2 {
             const int q = e % L, cl = e / L;
             const size_t off = (size_t)(k20 + cl) * n1 + q;
             cx<T> a, b;
-            ld_stream2(&Z[base + off * 2], a, b);
+            ld_z2<NTZ>(&Z[base + off * 2], a, b);
             tile[cl * LP + ldsgeom<L>::pad(q)] = a;
             tile[(TC + cl) * LP + ldsgeom<L>::pad(q)] = b;
         }
@@ -375,7 +432,8 @@ struct IO_SpinP2 {
         constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
         const size_t base = (size_t)blockIdx.y * n;
         const int k20 = blockIdx.x * TC;
-        const T aa = (T)sky[0], bb = (T)sky[1];
+        // F = |Fa|^2/aa + |Fb|^2/bb with the two reciprocals formed once
+        const T raa = (T)1 / (T)sky[0], rbb = (T)1 / (T)sky[1];
         const T thrT = (T)thr;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int k1 = e % L, cl = e / L;
@@ -384,8 +442,8 @@ struct IO_SpinP2 {
             bool c = false;
             T F = 0;
             if (k >= fr0 && k <= fr1) {
-                F = cnorm(tile[cl * LP + ldsgeom<L>::pad(k1)]) / aa +
-                    cnorm(tile[(TC + cl) * LP + ldsgeom<L>::pad(k1)]) / bb;
+                F = cnorm(tile[cl * LP + ldsgeom<L>::pad(k1)]) * raa +
+                    cnorm(tile[(TC + cl) * LP + ldsgeom<L>::pad(k1)]) * rbb;
                 FT[base + (size_t)k2 * n1 + k1] = F;
                 c = emit_cand && k > fr0 && k < fr1 - 1 && !(F < thrT);
             }
@@ -411,19 +469,19 @@ struct IO_SpinP2 {
 // memory and its last stage computing F in registers (no LDS round trip for
 // the load and the store).  One workgroup = one column k2, arrays a and b;
 // L in {512, 1024, 2048}: stages 16, 16, L/256.
-template <typename T, int L>
-__global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T> io, const cx<T> *__restrict__ twL)
+template <typename T, int L, bool NTZ>
+__global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL)
 {
     constexpr int NCOL = 2, NT = L / 8, RS = L / 256, NB = 16 / RS, LR = L / RS;
     __shared__ cx<T> tile[NCOL * ldsgeom<L>::LP];
     const size_t base = (size_t)blockIdx.y * 2 * io.n;
     const int k2 = blockIdx.x;
     const cx<T> *zc = io.Z + base + (size_t)k2 * io.n1 * 2;
-    stockham_first_from<T, -1, L, NCOL, true>(tile, [&](int c, int q) { return ld_stream(zc + (size_t)q * 2 + c); });
+    stockham_first_from<T, -1, L, NCOL, true>(tile, [&](int c, int q) { return ld_z<NTZ>(zc + (size_t)q * 2 + c); });
     stockham_stage<T, -1, L, NCOL, 16, 16>(tile, twL);
     cx<T> v[16];
     stockham_last_regs<T, -1, L, NCOL, RS, 256>(tile, twL, v);
-    const T aa = (T)io.sky[0], bb = (T)io.sky[1];
+    const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
     const T thrT = (T)io.thr;
     T *ft = io.FT + (size_t)blockIdx.y * io.n + (size_t)k2 * io.n1;
 #pragma unroll
@@ -436,7 +494,7 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T> io, con
             bool c = false;
             T F = 0;
             if (k >= io.fr0 && k <= io.fr1) {
-                F = cnorm(v[u * RS + r]) / aa + cnorm(v[(u + NB / 2) * RS + r]) / bb;
+                F = cnorm(v[u * RS + r]) * raa + cnorm(v[(u + NB / 2) * RS + r]) * rbb;
                 ft[k1] = F;
                 c = io.emit_cand && k > io.fr0 && k < io.fr1 - 1 && !(F < thrT);
             }
@@ -960,15 +1018,26 @@ hipError_t launch_splint(const void *Y, const void *Y2, const double *shftf, con
     return hipGetLastError();
 }
 
-template <typename T, int NIFO, bool HALF>
+template <typename T, int NIFO, bool HALF, bool NTZ>
 static hipError_t launch_spin_pass1_n(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
-    IO_SpinP1<T, NIFO, HALF> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z,
-                          (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D], (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D],
-                          s.N, p.n, p.n1, s.nifo, s.slot0, s.zero_cnt, s.nzero};
+    IO_SpinP1<T, NIFO, HALF, NTZ> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z,
+                                     (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D], (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D],
+                                     s.N, p.n, p.n1, s.nifo, s.slot0, s.zero_cnt, s.nzero};
     const int tc1 = p.e1 / p.n2 / 2;
     return launch_pass<T, -1>(p.n2, p.e1, io, dim3(p.n1 / tc1, s.nslots),
                               (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2], st);
+}
+
+template <typename T, bool HALF, bool NTZ>
+static hipError_t launch_spin_pass1_h(const FftPlan &p, const SpinArgs &s, hipStream_t st)
+{
+    switch (s.nifo) {
+    case 1: return launch_spin_pass1_n<T, 1, HALF, NTZ>(p, s, st);
+    case 2: return launch_spin_pass1_n<T, 2, HALF, NTZ>(p, s, st);
+    case 3: return launch_spin_pass1_n<T, 3, HALF, NTZ>(p, s, st);
+    default: return launch_spin_pass1_n<T, 0, HALF, NTZ>(p, s, st);
+    }
 }
 
 template <typename T>
@@ -976,40 +1045,28 @@ hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st
 {
     // N <= n1 * n2/2: the upper half of every pass-1 column is zero padding
     const bool half = (long long)s.N <= (long long)p.n1 * (p.n2 / 2);
-    if (half) {
-        switch (s.nifo) {
-        case 1: return launch_spin_pass1_n<T, 1, true>(p, s, st);
-        case 2: return launch_spin_pass1_n<T, 2, true>(p, s, st);
-        case 3: return launch_spin_pass1_n<T, 3, true>(p, s, st);
-        default: return launch_spin_pass1_n<T, 0, true>(p, s, st);
-        }
-    }
-    switch (s.nifo) {
-    case 1: return launch_spin_pass1_n<T, 1, false>(p, s, st);
-    case 2: return launch_spin_pass1_n<T, 2, false>(p, s, st);
-    case 3: return launch_spin_pass1_n<T, 3, false>(p, s, st);
-    default: return launch_spin_pass1_n<T, 0, false>(p, s, st);
-    }
+    if (half) return p.ntz ? launch_spin_pass1_h<T, true, true>(p, s, st) : launch_spin_pass1_h<T, true, false>(p, s, st);
+    return p.ntz ? launch_spin_pass1_h<T, false, true>(p, s, st) : launch_spin_pass1_h<T, false, false>(p, s, st);
 }
 
-template <typename T>
-hipError_t launch_spin_pass2(const FftPlan &p, const SpinArgs &s, hipStream_t st)
+template <typename T, bool NTZ>
+static hipError_t launch_spin_pass2_z(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
-    IO_SpinP2<T> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1,
-                    s.emit_cand, s.slot0, s.thr, s.cand, s.cand_cnt, s.cand_cap};
+    IO_SpinP2<T, NTZ> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1,
+                         s.emit_cand, s.slot0, s.thr, s.cand, s.cand_cnt, s.cand_cap};
     if (p.f2 && p.e2 == 2 * p.n1) {
         const cx<T> *tw = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
         const dim3 grid(p.n2, s.nslots);
         switch (p.n1) {
         case 512:
-            hipLaunchKernelGGL((k_spin_pass2_fused<T, 512>), grid, dim3(64), 0, st, io, tw);
+            hipLaunchKernelGGL((k_spin_pass2_fused<T, 512, NTZ>), grid, dim3(64), 0, st, io, tw);
             return hipGetLastError();
         case 1024:
-            hipLaunchKernelGGL((k_spin_pass2_fused<T, 1024>), grid, dim3(128), 0, st, io, tw);
+            hipLaunchKernelGGL((k_spin_pass2_fused<T, 1024, NTZ>), grid, dim3(128), 0, st, io, tw);
             return hipGetLastError();
         case 2048:
             if constexpr (sizeof(T) == 4) {
-                hipLaunchKernelGGL((k_spin_pass2_fused<T, 2048>), grid, dim3(256), 0, st, io, tw);
+                hipLaunchKernelGGL((k_spin_pass2_fused<T, 2048, NTZ>), grid, dim3(256), 0, st, io, tw);
                 return hipGetLastError();
             }
             break;
@@ -1020,6 +1077,12 @@ hipError_t launch_spin_pass2(const FftPlan &p, const SpinArgs &s, hipStream_t st
     const int tc2 = p.e2 / p.n1 / 2;
     return launch_pass<T, -1>(p.n1, p.e2, io, dim3(p.n2 / tc2, s.nslots),
                               (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1], st);
+}
+
+template <typename T>
+hipError_t launch_spin_pass2(const FftPlan &p, const SpinArgs &s, hipStream_t st)
+{
+    return p.ntz ? launch_spin_pass2_z<T, true>(p, s, st) : launch_spin_pass2_z<T, false>(p, s, st);
 }
 
 template <typename T>
