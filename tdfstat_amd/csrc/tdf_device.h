@@ -57,6 +57,47 @@ __device__ __forceinline__ void ld_stream2(const cx<T> *p, cx<T> &a, cx<T> &b)
     a = cx<T>{v.x, v.y};
     b = cx<T>{v.z, v.w};
 }
+// Buffer resources (raw buffer loads): word 3 of the descriptor for gfx9
+// parts, and the cache-policy bit of a non-temporal access.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+constexpr int BUF_NT = 2;
+
+// Raw buffer accesses with a 32-bit per-lane offset vo and a uniform
+// offset so (bytes); AUX is the cache policy (0 or BUF_NT).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, size_t bytes)
+{
+    const int nrec = bytes > 0x7fffffffull ? 0x7fffffff : (int)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, nrec, BUF_RSRC_WORD3);
+}
+template <int AUX, typename T>
+__device__ __forceinline__ cx<T> buf_ld_cx(__amdgpu_buffer_rsrc_t r, int vo, int so)
+{
+    if constexpr (sizeof(T) == 4) return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, AUX));
+    else return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, AUX));
+}
+template <int AUX>
+__device__ __forceinline__ double buf_ld_f64(__amdgpu_buffer_rsrc_t r, int vo, int so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, AUX));
+}
+// Two adjacent complex values (a, b) at byte offset vo.  Stores take no
+// scalar offset: for a store of more than 8 bytes with a register in the
+// scalar-offset field the compiler schedules VALU writes to the data
+// registers right after the store, and on gfx950 such stores were seen to
+// write corrupted data now and then; with offset 0 the wait state is kept.
+template <int AUX, typename T>
+__device__ __forceinline__ void buf_st_cx2(__amdgpu_buffer_rsrc_t r, int vo, cx<T> a, cx<T> b)
+{
+    using i4 = int __attribute__((ext_vector_type(4)));
+    if constexpr (sizeof(T) == 4) {
+        const vec4_t<T> v = {a.x, a.y, b.x, b.y};
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i4, v), r, vo, 0, AUX);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i4, a), r, vo, 0, AUX);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i4, b), r, vo + 16, 0, AUX);
+    }
+}
+
 // Z (the four-step intermediate) accesses: non-temporal (NT) when a launch's
 // Z exceeds the memory-side cache and goes to HBM, ordinary when it fits
 // there and the next kernel reads it back from that cache.
@@ -279,6 +320,22 @@ struct ldsgeom {
     __device__ static __forceinline__ int pad(int p) { return p + (p >> 4); }
 };
 
+// LDS index of position p + r*S of a column (base index cb = column*LP):
+// when S is a multiple of 16 the padding of p + r*S is pad(p) + r*S/16, so
+// the index is a per-lane base plus a compile-time step (an instruction
+// offset of the LDS access) instead of a per-access computation.
+template <int L, int S>
+struct ldsseq {
+    static constexpr bool LIN = S % 16 == 0;
+    static constexpr int STEP = S + S / 16;
+    __device__ static __forceinline__ int base(int cb, int p) { return cb + ldsgeom<L>::pad(p); }
+    __device__ static __forceinline__ int at(int cb, int p, int b, int r)
+    {
+        if constexpr (LIN) return b + r * STEP;
+        else return cb + ldsgeom<L>::pad(p + r * S);
+    }
+};
+
 // One Stockham radix-RS stage (NS = product of the earlier radices) over all
 // NCOL columns.  Each of the NT = NCOL*L/16 threads owns 16/RS butterflies.
 // twL[m] = exp(-2 pi i m / L).
@@ -301,12 +358,15 @@ __device__ __forceinline__ void stockham_stage(cx<T> *tile, const cx<T> *__restr
             for (int r = 1; r < RS; r++) w[u * RS + r] = twL[r * k * (L / (NS * RS))];
         }
     }
+    using RD = ldsseq<L, LR>;
+    using WR = ldsseq<L, NS>;
 #pragma unroll
     for (int u = 0; u < NB; u++) {
         const int g = tid + u * NT;
         const int c = g / LR, j = g % LR;
+        const int rb = RD::base(c * LP, j);
 #pragma unroll
-        for (int r = 0; r < RS; r++) v[u * RS + r] = tile[c * LP + ldsgeom<L>::pad(j + r * LR)];
+        for (int r = 0; r < RS; r++) v[u * RS + r] = tile[RD::at(c * LP, j, rb, r)];
     }
     __syncthreads();
 #pragma unroll
@@ -324,8 +384,13 @@ __device__ __forceinline__ void stockham_stage(cx<T> *tile, const cx<T> *__restr
         }
         dftR<RS, SIGN>(&v[u * RS]);
         const int idxD = (j / NS) * NS * RS + k;
+        const int wb = WR::base(c * LP, idxD);
 #pragma unroll
-        for (int r = 0; r < RS; r++) tile[c * LP + ldsgeom<L>::pad(idxD + r * NS)] = v[u * RS + r];
+        for (int r = 0; r < RS; r++) {
+            // NS = 1, RS = 16: positions 16j + r, r < 16, share one pad
+            const int a = (NS == 1 && RS == 16) ? wb + r : WR::at(c * LP, idxD, wb, r);
+            tile[a] = v[u * RS + r];
+        }
     }
     __syncthreads();
 }
@@ -348,9 +413,27 @@ __device__ __forceinline__ void stockham_first_from(cx<T> *tile, LD ld)
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = ld(c, j + r * LR);
     dftR<16, SIGN>(v);
+    const int wb = c * LP + 17 * j;  // pad(16j + r) = 17j + r for r < 16
 #pragma unroll
-    for (int r = 0; r < 16; r++) tile[c * LP + ldsgeom<L>::pad(16 * j + r)] = v[r];
+    for (int r = 0; r < 16; r++) tile[wb + r] = v[r];
     (void)NT;
+    __syncthreads();
+}
+
+// The same first stage with its 16 inputs already in registers: v[r] is
+// position j + r*L/16 of column c (c, j as in stockham_first_from).
+template <typename T, int SIGN, int L, int NCOL, bool PAIRED = false>
+__device__ __forceinline__ void stockham_first_regs(cx<T> *tile, cx<T> *v)
+{
+    constexpr int LR = L / 16;
+    constexpr int LP = ldsgeom<L>::LP;
+    static_assert(!PAIRED || NCOL == 2, "paired columns");
+    const int g = threadIdx.x;
+    const int c = PAIRED ? (g & 1) : g / LR, j = PAIRED ? (g >> 1) : g % LR;
+    dftR<16, SIGN>(v);
+    const int wb = c * LP + 17 * j;  // pad(16j + r) = 17j + r for r < 16
+#pragma unroll
+    for (int r = 0; r < 16; r++) tile[wb + r] = v[r];
     __syncthreads();
 }
 
@@ -372,12 +455,14 @@ __device__ __forceinline__ void stockham_last_regs(const cx<T> *tile, const cx<T
 #pragma unroll
         for (int r = 1; r < RS; r++) w[u * RS + r] = twL[r * k * (L / (NS * RS))];
     }
+    using RD = ldsseq<L, LR>;
 #pragma unroll
     for (int u = 0; u < NB; u++) {
         const int g = tid + u * NT;
         const int c = g / LR, j = g % LR;
+        const int rb = RD::base(c * LP, j);
 #pragma unroll
-        for (int r = 0; r < RS; r++) v[u * RS + r] = tile[c * LP + ldsgeom<L>::pad(j + r * LR)];
+        for (int r = 0; r < RS; r++) v[u * RS + r] = tile[RD::at(c * LP, j, rb, r)];
     }
 #pragma unroll
     for (int u = 0; u < NB; u++) {
@@ -398,15 +483,18 @@ __device__ __forceinline__ void stockham_first_half(cx<T> *tile)
 {
     constexpr int LR = L / 16;
     constexpr int LP = ldsgeom<L>::LP;
+    using RD = ldsseq<L, LR>;
     const int g = threadIdx.x;
     const int c = g / LR, j = g % LR;
     cx<T> v[16];
+    const int rb = RD::base(c * LP, j);
 #pragma unroll
-    for (int r = 0; r < 8; r++) v[r] = tile[c * LP + ldsgeom<L>::pad(j + r * LR)];
+    for (int r = 0; r < 8; r++) v[r] = tile[RD::at(c * LP, j, rb, r)];
     __syncthreads();
     dft16_half<SIGN>(v);
+    const int wb = c * LP + 17 * j;  // pad(16j + r) = 17j + r for r < 16
 #pragma unroll
-    for (int r = 0; r < 16; r++) tile[c * LP + ldsgeom<L>::pad(16 * j + r)] = v[r];
+    for (int r = 0; r < 16; r++) tile[wb + r] = v[r];
     __syncthreads();
 }
 

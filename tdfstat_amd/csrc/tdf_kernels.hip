@@ -309,6 +309,10 @@ struct IO_SpinP1 {
             constexpr int KB0 = 8 / NIFO >= 8 ? 8 : 8 / NIFO >= 4 ? 4 : 8 / NIFO >= 2 ? 2 : 1;
             constexpr int KB = KB0 < NIT ? KB0 : NIT;
             static_assert(NIT % KB == 0, "batch");
+            // buffer resources based at this tile's first column: 32-bit
+            // lane offsets, the detector planes as uniform offsets
+            const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xmb, ((size_t)2 * NIFO * n - (size_t)c0 * L) * sizeof(cx<T>));
+            const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tmb, ((size_t)NIFO * n - (size_t)c0 * L) * sizeof(double));
 #pragma unroll
             for (int k0 = 0; k0 < NIT; k0 += KB) {
                 double t1v[KB][NIFO];
@@ -319,9 +323,9 @@ struct IO_SpinP1 {
                     const int off = (e / LQ) * L + e % LQ;
 #pragma unroll
                     for (int d = 0; d < NIFO; d++) {
-                        t1v[u][d] = tmb[(size_t)d * n + off];
-                        mav[u][d] = xmb[(size_t)(2 * d) * n + off];
-                        mbv[u][d] = xmb[(size_t)(2 * d + 1) * n + off];
+                        t1v[u][d] = buf_ld_f64<0>(rt, off * (int)sizeof(double), d * n * (int)sizeof(double));
+                        mav[u][d] = buf_ld_cx<0, T>(rx, off * (int)sizeof(cx<T>), 2 * d * n * (int)sizeof(cx<T>));
+                        mbv[u][d] = buf_ld_cx<0, T>(rx, off * (int)sizeof(cx<T>), (2 * d + 1) * n * (int)sizeof(cx<T>));
                     }
                 }
 #pragma unroll
@@ -386,10 +390,19 @@ struct IO_SpinP1 {
         const int c = c0 + cl;
         cx<T> w = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * k20));
         const cx<T> ws = twiddle_n<SIGN>(tw_lo, tw_hi, (unsigned)(c * (NT / TC)));
+        // Z of this template slot through a buffer resource with 32-bit
+        // lane offsets
+        const __amdgpu_buffer_rsrc_t rz = buf_rsrc(Z + base, (size_t)2 * n * sizeof(cx<T>));
+        const int vo = (k20 * n1 + c) * 2 * (int)sizeof(cx<T>);
+        const int ta = cl * LP + ldsgeom<L>::pad(k20), tb = (TC + cl) * LP + ldsgeom<L>::pad(k20);
+        constexpr int RSTEP = NT / TC;
 #pragma unroll
-        for (int k2 = k20; k2 < L; k2 += NT / TC) {
-            st_z2<NTZ>(&Z[base + ((size_t)k2 * n1 + c) * 2], cmul(tile[cl * LP + ldsgeom<L>::pad(k2)], w),
-                       cmul(tile[(TC + cl) * LP + ldsgeom<L>::pad(k2)], w));
+        for (int u = 0; u * RSTEP < L; u++) {
+            const int ro = u * RSTEP * n1 * 2 * (int)sizeof(cx<T>);  // row step, in the lane offset
+            // pad(k20 + u*RSTEP) - pad(k20) is u*RSTEP*17/16 for RSTEP a multiple of 16
+            const int lo = RSTEP % 16 == 0 ? u * (RSTEP + RSTEP / 16)
+                                           : ldsgeom<L>::pad(k20 + u * RSTEP) - ldsgeom<L>::pad(k20);
+            buf_st_cx2<NTZ ? BUF_NT : 0, T>(rz, vo + ro, cmul(tile[ta + lo], w), cmul(tile[tb + lo], w));
             w = cmul(w, ws);
         }
     }
@@ -465,25 +478,42 @@ struct IO_SpinP2 {
     }
 };
 
-// Pass 2 with its first Stockham stage reading Z directly from global
-// memory and its last stage computing F in registers (no LDS round trip for
-// the load and the store).  One workgroup = one column k2, arrays a and b;
-// L in {512, 1024, 2048}: stages 16, 16, L/256.
+// Pass 2 with its first Stockham stage taking Z from registers (loaded
+// straight from global memory) and its last stage computing F in registers
+// (no LDS round trip for the load and the store).  A column is k2 of
+// template slot, arrays a and b; L in {512, 1024, 2048}: stages 16, 16,
+// L/256.  PAIRED lanes: lane g holds array g & 1 at rows (g >> 1) + r*L/16.
+// The column's 2*L values are read through a buffer resource based at the
+// column (one offset register per lane, the row step r*L/16 as a scalar
+// offset), so the loads need no per-load 64-bit addresses.
 template <typename T, int L, bool NTZ>
-__global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL)
+__device__ __forceinline__ void pass2_load(const IO_SpinP2<T, NTZ> &io, int slot, int k2, cx<T> *v)
+{
+    const cx<T> *zc = io.Z + (size_t)slot * 2 * io.n + (size_t)k2 * io.n1 * 2;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)zc, (short)0, (int)(2 * L * sizeof(cx<T>)), BUF_RSRC_WORD3);
+    const int c = threadIdx.x & 1, j = threadIdx.x >> 1;
+    const int vo = (2 * j + c) * (int)sizeof(cx<T>);
+    constexpr int AUX = NTZ ? BUF_NT : 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int so = r * (L / 16) * 2 * (int)sizeof(cx<T>);
+        if constexpr (sizeof(T) == 4) v[r] = __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, AUX));
+        else v[r] = __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(rs, vo, so, AUX));
+    }
+}
+
+template <typename T, int L, bool NTZ>
+__device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const cx<T> *__restrict__ twL,
+                                             cx<T> *tile, int slot, int k2, cx<T> *v)
 {
     constexpr int NCOL = 2, NT = L / 8, RS = L / 256, NB = 16 / RS, LR = L / RS;
-    __shared__ cx<T> tile[NCOL * ldsgeom<L>::LP];
-    const size_t base = (size_t)blockIdx.y * 2 * io.n;
-    const int k2 = blockIdx.x;
-    const cx<T> *zc = io.Z + base + (size_t)k2 * io.n1 * 2;
-    stockham_first_from<T, -1, L, NCOL, true>(tile, [&](int c, int q) { return ld_z<NTZ>(zc + (size_t)q * 2 + c); });
+    stockham_first_regs<T, -1, L, NCOL, true>(tile, v);
     stockham_stage<T, -1, L, NCOL, 16, 16>(tile, twL);
-    cx<T> v[16];
     stockham_last_regs<T, -1, L, NCOL, RS, 256>(tile, twL, v);
     const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
     const T thrT = (T)io.thr;
-    T *ft = io.FT + (size_t)blockIdx.y * io.n + (size_t)k2 * io.n1;
+    T *ft = io.FT + (size_t)slot * io.n + (size_t)k2 * io.n1;
 #pragma unroll
     for (int u = 0; u < NB / 2; u++) {
         const int j = (threadIdx.x + u * NT) % LR;
@@ -508,13 +538,22 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io
                     bpos = __shfl(bpos, leader, 64);
                     if (c) {
                         const int pos = bpos + __popcll(m & ((1ull << lane) - 1ull));
-                        if (pos < io.cand_cap) io.cand[pos] = TrigEntry{io.slot0 + (int)blockIdx.y, k, 0.f, (float)F};
+                        if (pos < io.cand_cap) io.cand[pos] = TrigEntry{io.slot0 + slot, k, 0.f, (float)F};
                     }
                 }
             }
         }
     }
-    (void)LR;
+}
+
+// One workgroup per column: grid (n2, slots).
+template <typename T, int L, bool NTZ>
+__global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL)
+{
+    __shared__ cx<T> tile[2 * ldsgeom<L>::LP];
+    cx<T> v[16];
+    pass2_load<T, L, NTZ>(io, blockIdx.y, blockIdx.x, v);
+    pass2_column<T, L, NTZ>(io, twL, tile, blockIdx.y, blockIdx.x, v);
 }
 
 // ---------------------------------------------------------------------
