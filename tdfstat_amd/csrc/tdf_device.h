@@ -300,6 +300,35 @@ __device__ __forceinline__ void dft16_half(cx<T> *v)
     }
 }
 
+// Outputs 0 .. R/2-1 of the R-point DFT only (v[R/2..R-1] are left as
+// scratch): the lower half of the spectrum of a last stage.
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dftR_lo(cx<T> *v)
+{
+    if constexpr (R == 2) {
+        v[0] = v[0] + v[1];
+    } else if constexpr (R == 4) {
+        const cx<T> a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = mul_si<SIGN>(v[1] - v[3]);
+        v[0] = a0 + a2;
+        v[1] = a1 + a3;
+    } else if constexpr (R == 8) {
+        const T h = (T)0.70710678118654752440;
+        cx<T> a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+        cx<T> b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+        dft4s<SIGN>(a0, a1, a2, a3);
+        dft4s<SIGN>(b0, b1, b2, b3);
+        b1 = cmul(b1, cx<T>{h, (T)SIGN * h});
+        b2 = mul_si<SIGN>(b2);
+        b3 = cmul(b3, cx<T>{-h, (T)SIGN * h});
+        v[0] = a0 + b0;
+        v[1] = a1 + b1;
+        v[2] = a2 + b2;
+        v[3] = a3 + b3;
+    } else {
+        dft16<SIGN>(v);
+    }
+}
+
 template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dftR(cx<T> *v)
 {
@@ -440,7 +469,9 @@ __device__ __forceinline__ void stockham_first_regs(cx<T> *tile, cx<T> *v)
 // Last Stockham stage (radix RS, NS) leaving its outputs in registers:
 // v[u*RS + r] is output position idxD(u) + r*NS of column c(u), with
 // g = tid + u*NT, c = g / (L/RS), j = g % (L/RS), idxD = (j/NS)*NS*RS + j%NS.
-template <typename T, int SIGN, int L, int NCOL, int RS, int NS>
+// LO: only outputs r < RS/2 of every butterfly are formed (v[u*RS + r],
+// r >= RS/2, are scratch), for a caller that needs positions below L/2.
+template <typename T, int SIGN, int L, int NCOL, int RS, int NS, bool LO = false>
 __device__ __forceinline__ void stockham_last_regs(const cx<T> *tile, const cx<T> *__restrict__ twL, cx<T> *v)
 {
     constexpr int NT = NCOL * L / 16;
@@ -472,7 +503,8 @@ __device__ __forceinline__ void stockham_last_regs(const cx<T> *tile, const cx<T
             if constexpr (SIGN > 0) wr.y = -wr.y;
             v[u * RS + r] = cmul(v[u * RS + r], wr);
         }
-        dftR<RS, SIGN>(&v[u * RS]);
+        if constexpr (LO) dftR_lo<RS, SIGN>(&v[u * RS]);
+        else dftR<RS, SIGN>(&v[u * RS]);
     }
 }
 

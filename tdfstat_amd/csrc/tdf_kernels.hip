@@ -503,14 +503,17 @@ __device__ __forceinline__ void pass2_load(const IO_SpinP2<T, NTZ> &io, int slot
     }
 }
 
-template <typename T, int L, bool NTZ>
+// LO: the band ends below n/2 (fr1 < n/2), so only k1 < L/2 (r < RS/2)
+// can hold bins of the band and the last stage forms only those outputs.
+template <typename T, int L, bool NTZ, bool LO>
 __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const cx<T> *__restrict__ twL,
                                              cx<T> *tile, int slot, int k2, cx<T> *v)
 {
     constexpr int NCOL = 2, NT = L / 8, RS = L / 256, NB = 16 / RS, LR = L / RS;
+    constexpr int RO = LO ? RS / 2 : RS;  // outputs per butterfly that are used
     stockham_first_regs<T, -1, L, NCOL, true>(tile, v);
     stockham_stage<T, -1, L, NCOL, 16, 16>(tile, twL);
-    stockham_last_regs<T, -1, L, NCOL, RS, 256>(tile, twL, v);
+    stockham_last_regs<T, -1, L, NCOL, RS, 256, LO>(tile, twL, v);
     const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
     const T thrT = (T)io.thr;
     T *ft = io.FT + (size_t)slot * io.n + (size_t)k2 * io.n1;
@@ -518,7 +521,7 @@ __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const 
     for (int u = 0; u < NB / 2; u++) {
         const int j = (threadIdx.x + u * NT) % LR;
 #pragma unroll
-        for (int r = 0; r < RS; r++) {
+        for (int r = 0; r < RO; r++) {
             const int k1 = j + r * 256;
             const int k = k2 + io.n2 * k1;
             bool c = false;
@@ -547,13 +550,13 @@ __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const 
 }
 
 // One workgroup per column: grid (n2, slots).
-template <typename T, int L, bool NTZ>
+template <typename T, int L, bool NTZ, bool LO>
 __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL)
 {
     __shared__ cx<T> tile[2 * ldsgeom<L>::LP];
     cx<T> v[16];
     pass2_load<T, L, NTZ>(io, blockIdx.y, blockIdx.x, v);
-    pass2_column<T, L, NTZ>(io, twL, tile, blockIdx.y, blockIdx.x, v);
+    pass2_column<T, L, NTZ, LO>(io, twL, tile, blockIdx.y, blockIdx.x, v);
 }
 
 // ---------------------------------------------------------------------
@@ -1088,6 +1091,27 @@ hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st
     return p.ntz ? launch_spin_pass1_h<T, false, true>(p, s, st) : launch_spin_pass1_h<T, false, false>(p, s, st);
 }
 
+template <typename T, bool NTZ, bool LO>
+static hipError_t launch_fused(int n1, dim3 grid, const IO_SpinP2<T, NTZ> &io, const cx<T> *tw, hipStream_t st)
+{
+    switch (n1) {
+    case 512:
+        hipLaunchKernelGGL((k_spin_pass2_fused<T, 512, NTZ, LO>), grid, dim3(64), 0, st, io, tw);
+        return hipGetLastError();
+    case 1024:
+        hipLaunchKernelGGL((k_spin_pass2_fused<T, 1024, NTZ, LO>), grid, dim3(128), 0, st, io, tw);
+        return hipGetLastError();
+    case 2048:
+        if constexpr (sizeof(T) == 4) {
+            hipLaunchKernelGGL((k_spin_pass2_fused<T, 2048, NTZ, LO>), grid, dim3(256), 0, st, io, tw);
+            return hipGetLastError();
+        }
+        break;
+    default: break;
+    }
+    return hipErrorInvalidValue;
+}
+
 template <typename T, bool NTZ>
 static hipError_t launch_spin_pass2_z(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
@@ -1096,22 +1120,9 @@ static hipError_t launch_spin_pass2_z(const FftPlan &p, const SpinArgs &s, hipSt
     if (p.f2 && p.e2 == 2 * p.n1) {
         const cx<T> *tw = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
         const dim3 grid(p.n2, s.nslots);
-        switch (p.n1) {
-        case 512:
-            hipLaunchKernelGGL((k_spin_pass2_fused<T, 512, NTZ>), grid, dim3(64), 0, st, io, tw);
-            return hipGetLastError();
-        case 1024:
-            hipLaunchKernelGGL((k_spin_pass2_fused<T, 1024, NTZ>), grid, dim3(128), 0, st, io, tw);
-            return hipGetLastError();
-        case 2048:
-            if constexpr (sizeof(T) == 4) {
-                hipLaunchKernelGGL((k_spin_pass2_fused<T, 2048, NTZ>), grid, dim3(256), 0, st, io, tw);
-                return hipGetLastError();
-            }
-            break;
-        default: break;
-        }
-        return hipErrorInvalidValue;
+        // bins k = k2 + n2*k1 < n/2 have k1 < n1/2
+        if (s.fr1 < p.n / 2) return launch_fused<T, NTZ, true>(p.n1, grid, io, tw, st);
+        return launch_fused<T, NTZ, false>(p.n1, grid, io, tw, st);
     }
     const int tc2 = p.e2 / p.n1 / 2;
     return launch_pass<T, -1>(p.n1, p.e2, io, dim3(p.n2 / tc2, s.nslots),
