@@ -259,7 +259,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="max spindowns per launch (0 = library default)")
     ap.add_argument("--no-tune", action="store_true", help="use the default plan instead of tdf_autotune")
     ap.add_argument("--plan", default=None,
-                    help="FP32 plan n1,e1,e2,fused,batch[,ntz] instead of tdf_autotune (profiling runs)")
+                    help="FP32 plan n1,e1,e2,fused,batch[,ntz[,stream]] instead of tdf_autotune (profiling runs)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (profiling runs)")
     ap.add_argument("--per-step", type=int, default=0,
                     help="sky points per step (one tdf_job_core_batch call); 1 = one tdf_job_core call per step; "
@@ -291,8 +291,8 @@ def main():
           f"sky={len(pts)} S={S}", file=sys.stderr, flush=True)
 
     plan = tuple(int(v) for v in args.plan.split(",")) if args.plan else None
-    if plan is not None and len(plan) not in (5, 6):
-        ap.error("--plan takes n1,e1,e2,fused,batch[,ntz]")
+    if plan is not None and len(plan) not in (5, 6, 7):
+        ap.error("--plan takes n1,e1,e2,fused,batch[,ntz[,stream]]")
     per_step = args.per_step or cfg["per_step"]
     ctx32, opts, nt, el, kt, ntrig, ntp = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
                                              args.batch, not args.no_tune, plan, per_step)

@@ -251,7 +251,9 @@ int tdf_autotune(tdf_ctx *ctx);
  * e1/e2 of the two passes, fused = bit 0: pass 2 reads Z and writes F
  * directly from registers, bit 1: the four-step intermediate Z is written
  * and read with non-temporal accesses (set when a launch's Z goes to HBM,
- * clear when it fits the memory-side cache), batch = spindowns per launch */
+ * clear when it fits the memory-side cache), bit 2 (FP32, with bit 0): the
+ * fused pass 2 as a streaming grid that stages the next column in LDS while
+ * it transforms the current one, batch = spindowns per launch */
 int tdf_plan_info(tdf_ctx *ctx, int *n1, int *e1, int *e2, int *fused, int *batch);
 /* selects a plan explicitly (same meaning; batch <= the creation batch) */
 int tdf_set_plan(tdf_ctx *ctx, int n1, int e1, int e2, int fused, int batch);

@@ -559,6 +559,61 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io
     pass2_column<T, L, NTZ, LO>(io, twL, tile, blockIdx.y, blockIdx.x, v);
 }
 
+// Streaming form (FP32): a grid of resident workgroups walks the columns
+// item = k2 + n2*slot (stride gridDim.x).  A column's Z (2*L complex,
+// rows q with a, b interleaved) is copied by the memory system straight
+// into an LDS staging area (buffer loads to LDS, 1 KiB per wavefront
+// instruction); the copy of the next column is issued as soon as the
+// current one has been read into registers, so it runs while the column is
+// transformed.
+template <int L>
+struct stream_geom {
+    static constexpr int NT = L / 8, W = NT / 64;
+    static constexpr int CHUNKS = 2 * L * 8 / 1024;  // 1 KiB chunks of a column
+    static constexpr int PER_WAVE = CHUNKS / W;      // = 8
+};
+
+template <typename T, int L, bool NTZ>
+__device__ __forceinline__ void stream_issue(const IO_SpinP2<T, NTZ> &io, int item, cx<T> *raw)
+{
+    using G = stream_geom<L>;
+    const int slot = item / io.n2, k2 = item % io.n2;
+    const cx<T> *zc = io.Z + (size_t)slot * 2 * io.n + (size_t)k2 * io.n1 * 2;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(zc, (size_t)2 * L * sizeof(cx<T>));
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < G::PER_WAVE; i++) {
+        const int m = w * G::PER_WAVE + i;  // chunk: rows 64m .. 64m+63
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)(raw + 128 * m), 16,
+                                                 lane * 16, m * 1024, 0, NTZ ? BUF_NT : 0);
+    }
+}
+
+template <typename T, int L, bool NTZ, bool LO>
+__global__ void __launch_bounds__(L / 8) k_spin_pass2_stream(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL,
+                                                             int nitems)
+{
+    static_assert(sizeof(T) == 4, "FP32 columns of 16 bytes per row");
+    __shared__ cx<T> tile[2 * ldsgeom<L>::LP];
+    __shared__ cx<T> raw[2 * L];
+    int item = blockIdx.x;
+    if (item < nitems) stream_issue<T, L, NTZ>(io, item, raw);
+    const int c = threadIdx.x & 1, j = threadIdx.x >> 1;
+    for (; item < nitems; item += gridDim.x) {
+        // this wavefront's copies (and earlier stores) are done; the barrier
+        // makes every wavefront's part visible
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        cx<T> v[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = raw[2 * (j + r * (L / 16)) + c];
+        __syncthreads();  // the staging area is read: the next copy may land
+        const int nxt = item + gridDim.x;
+        if (nxt < nitems) stream_issue<T, L, NTZ>(io, nxt, raw);
+        pass2_column<T, L, NTZ, LO>(io, twL, tile, item / io.n2, item % io.n2, v);
+    }
+}
+
 // ---------------------------------------------------------------------
 // Batched transpose of column-major FFT output: in[item][k2*n1 + k1]
 // (rows k2 < n2, cols k1 < n1) -> natural bin k = k2 + n2*k1, written to
@@ -1091,6 +1146,39 @@ hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st
     return p.ntz ? launch_spin_pass1_h<T, false, true>(p, s, st) : launch_spin_pass1_h<T, false, false>(p, s, st);
 }
 
+// Streaming pass 2: as many workgroups as are resident at once on the
+// device (occupancy x compute units), found once per kernel variant.
+template <typename T, int L, bool NTZ, bool LO>
+static hipError_t launch_stream_L(const IO_SpinP2<T, NTZ> &io, const cx<T> *tw, int nitems, hipStream_t st)
+{
+    static int resident = 0;
+    if (resident <= 0) {
+        int nb = 0, dev = 0, ncu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spin_pass2_stream<T, L, NTZ, LO>, L / 8, 0);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        resident = (nb > 0 ? nb : 1) * (ncu > 0 ? ncu : 1);
+    }
+    const int grid = resident < nitems ? resident : nitems;
+    hipLaunchKernelGGL((k_spin_pass2_stream<T, L, NTZ, LO>), dim3(grid), dim3(L / 8), 0, st, io, tw, nitems);
+    return hipGetLastError();
+}
+
+template <typename T, bool NTZ, bool LO>
+static hipError_t launch_stream(int n1, const IO_SpinP2<T, NTZ> &io, const cx<T> *tw, int nitems, hipStream_t st)
+{
+    if constexpr (sizeof(T) == 4) {
+        switch (n1) {
+        case 512: return launch_stream_L<T, 512, NTZ, LO>(io, tw, nitems, st);
+        case 1024: return launch_stream_L<T, 1024, NTZ, LO>(io, tw, nitems, st);
+        case 2048: return launch_stream_L<T, 2048, NTZ, LO>(io, tw, nitems, st);
+        default: break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
 template <typename T, bool NTZ, bool LO>
 static hipError_t launch_fused(int n1, dim3 grid, const IO_SpinP2<T, NTZ> &io, const cx<T> *tw, hipStream_t st)
 {
@@ -1121,7 +1209,13 @@ static hipError_t launch_spin_pass2_z(const FftPlan &p, const SpinArgs &s, hipSt
         const cx<T> *tw = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
         const dim3 grid(p.n2, s.nslots);
         // bins k = k2 + n2*k1 < n/2 have k1 < n1/2
-        if (s.fr1 < p.n / 2) return launch_fused<T, NTZ, true>(p.n1, grid, io, tw, st);
+        const bool lo = s.fr1 < p.n / 2;
+        if (p.f2 == 2) {
+            const int nitems = p.n2 * s.nslots;
+            return lo ? launch_stream<T, NTZ, true>(p.n1, io, tw, nitems, st)
+                      : launch_stream<T, NTZ, false>(p.n1, io, tw, nitems, st);
+        }
+        if (lo) return launch_fused<T, NTZ, true>(p.n1, grid, io, tw, st);
         return launch_fused<T, NTZ, false>(p.n1, grid, io, tw, st);
     }
     const int tc2 = p.e2 / p.n1 / 2;
