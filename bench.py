@@ -103,7 +103,8 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
     sett = seg.sett
     S = cfg["nspin"]
     opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
-    ctx = TS.Context(sett, seg.dets, prec=prec, batch=max(batch, plan[4] if plan else 0))
+    # a search keeps no F arrays: F only at bins >= thr (the trigger rule's input)
+    ctx = TS.Context(sett, seg.dets, prec=prec, batch=max(batch, plan[4] if plan else 0), keep_fstat=False)
     if plan:
         ctx.set_plan(*plan)
         print(f"[bench] prec={prec} plan {ctx.plan_info()}", file=sys.stderr, flush=True)
@@ -159,6 +160,8 @@ def fp32_vs_fp64_error(cfg, seg, pt, opts, ctx32, ctx64, nss=8):
     bins with F64 >= 2 on nss spindowns, and trigger-set equality over the
     full spindown range of sky point pt."""
     S = cfg["nspin"]
+    ctx32.keep_fstat(True)
+    ctx64.keep_fstat(True)
     sky32, r32 = ctx32.job_core(*pt, opts, max_spindowns=S + 1)
     sky64, r64 = ctx64.job_core(*pt, opts, max_spindowns=S + 1)
     thr = opts.thr

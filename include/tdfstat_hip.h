@@ -266,6 +266,13 @@ int tdf_set_plan(tdf_ctx *ctx, int n1, int e1, int e2, int fused, int batch);
 int tdf_profile_enable(tdf_ctx *ctx, int on);
 int tdf_kernel_times(tdf_ctx *ctx, int slot, double *ms, long long *launches,
                      const char **name);
+/* on: the spindown kernels write F at every bin of the band into the
+ * context's F array (read back by tdf_debug_fstat); off (the default): with
+ * the candidate path and the fused pass 2, F is written only at bins with
+ * F >= thr, with one flag bit per bin, which is all the trigger rule reads.
+ * No reference counterpart (the reference keeps F in a static array,
+ * jobcore.c:420-421). */
+int tdf_keep_fstat(tdf_ctx *ctx, int on);
 int tdf_synchronize(tdf_ctx *ctx);
 
 #ifdef __cplusplus

@@ -114,6 +114,7 @@ SIGNATURES = {
     "tdf_profile_enable": (C.c_int, [_P, C.c_int]),
     "tdf_kernel_times": (C.c_int, [_P, C.c_int, _DP, C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_char_p)]),
+    "tdf_keep_fstat": (C.c_int, [_P, C.c_int]),
     "tdf_synchronize": (C.c_int, [_P]),
 }
 

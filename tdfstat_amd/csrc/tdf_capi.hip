@@ -96,6 +96,10 @@ struct tdf_ctx {
     int pm = 0;
     // spindown batch buffers
     void *Z = nullptr, *FT = nullptr, *mu = nullptr;
+    // F at every bin of the band (keep_ft, needed by tdf_debug_fstat), or
+    // only at bins >= thr with one flag bit per bin (cbits, [batch][n/64])
+    bool keep_ft = false;
+    unsigned long long *cbits = nullptr;
     int nblk_cap = 0;
     TrigEntry *trig_d = nullptr, *trig_h = nullptr;
     // counters: [0] triggers, [1..S] per template, then one candidate count
@@ -380,6 +384,8 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
     const FftPlan &p = c->p_spin;
     const tdf_geom &g = c->g;
     const int nblk = o->fstat_norm ? (o->fr1 - o->fr0) / o->nav : 0;
+    // sparse F needs the candidate path and the fused pass 2
+    const int sparse = (use_cand && !c->keep_ft && p.f2 && p.e2 == 2 * p.n1) ? 1 : 0;
     int batch_idx = 0;
     for (int b0 = t0; b0 < t0 + count; b0 += c->batch, batch_idx++) {
         const int nslots = std::min(c->batch, t0 + count - b0);
@@ -404,6 +410,8 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
         sa.cand_cap = (int)c->cand_cap;
         sa.zero_cnt = cnt_dev;
         sa.nzero = b0 == t0 ? nzero : 0;
+        sa.cbits = c->cbits;
+        sa.sparse = sparse;
         {
             Prof pr(c, KS_SPIN1);
             HIPCHK(launch_spin_pass1<T>(p, sa, st));
@@ -435,6 +443,8 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
         ta.cap = (int)std::min<long long>(trig_cap, 0x7fffffff);
         ta.lines = c->lines_d;
         ta.sk = sk;
+        ta.cbits = c->cbits;
+        ta.sparse = sparse;
         if (use_cand) {
             Prof pr(c, KS_TRIG);
             HIPCHK(launch_candidates<T>(c->FT, ta, c->cand_d, cand_cnt, (int)c->cand_cap, trig_out, cnt_dev, st));
@@ -652,6 +662,7 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
     if (hipMemset(c->sky_scal, 0, sizeof(double) * 4) != hipSuccess) TRY(TDF_ERR_HIP);
     TRY(dmalloc(c, &c->Z, cs * 2 * B * (size_t)g->nfftf));
     TRY(dmalloc(c, &c->FT, c->rsz * B * (size_t)g->nfftf));
+    TRY(dmalloc(c, (void **)&c->cbits, B * (size_t)g->nfftf / 8));
     c->nblk_cap = g->nfftf / 16 + 1;  // nav >= 16 in practice; checked per call
     TRY(dmalloc(c, &c->mu, c->rsz * B * (size_t)c->nblk_cap));
     TRY(ensure_counters(c, std::max(4096, (int)B)));
@@ -1206,6 +1217,7 @@ int tdf_debug_fstat(tdf_ctx *c, int iss, double *F)
     if (!c || !F) return TDF_ERR_ARG;
     const int slot = iss - c->last_batch_iss0;
     if (slot < 0 || slot >= c->last_batch_n) return TDF_ERR_ARG;
+    if (!c->keep_ft) return TDF_ERR_STATE;  // F is kept only at bins >= thr
     const FftPlan &p = c->p_spin;
     const size_t n = p.n;
     std::vector<double> ft(n);
@@ -1364,6 +1376,10 @@ int tdf_autotune(tdf_ctx *c)
                     sa.nslots = b;
                     sa.fr0 = 0;
                     sa.fr1 = n / 2 - 1;
+                    // pass 2 as in a search: F flags only (no bin reaches thr)
+                    sa.cbits = c->cbits;
+                    sa.sparse = c->keep_ft ? 0 : 1;
+                    sa.thr = 1e30;
                     // one warm-up pair, then the mean of three pairs
                     float t = 0.f;
                     for (int rep = 0; rep < 2; rep++) {
@@ -1602,6 +1618,14 @@ int tdf_kernel_times(tdf_ctx *c, int slot, double *ms, long long *launches, cons
     if (ms) *ms = c->kms[slot];
     if (launches) *launches = c->kcount[slot];
     if (name) *name = kslot_names[slot];
+    return TDF_OK;
+}
+
+int tdf_keep_fstat(tdf_ctx *c, int on)
+{
+    if (!c) return TDF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->keep_ft = on != 0;
     return TDF_OK;
 }
 

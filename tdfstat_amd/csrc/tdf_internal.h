@@ -160,6 +160,10 @@ struct SpinArgs {
     // counters cleared by workgroup (0, 0) of pass 1 (nzero = 0: none)
     int *zero_cnt;
     int nzero;
+    // sparse F (fused pass 2 with candidates): F is written only where
+    // F >= thr, with one bit per FT position in cbits ([slot][n/64])
+    unsigned long long *cbits;
+    int sparse;
 };
 
 struct FstatArgs {
@@ -172,6 +176,9 @@ struct TrigArgs {
     int fstat_norm, nblk, nav, nblk_cap, numlines, cap;
     const double *lines;
     SkyScal sk;
+    // sparse F: FT holds F only at the bins flagged in cbits (F >= thr)
+    const unsigned long long *cbits;
+    int sparse;
 };
 
 struct TrigEntry {

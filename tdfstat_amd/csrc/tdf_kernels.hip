@@ -424,6 +424,8 @@ struct IO_SpinP2 {
     TrigEntry *cand;
     int *cand_cnt;
     int cand_cap;
+    unsigned long long *cbits;  // fused pass 2 with sparse F only
+    int sparse;
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
@@ -517,6 +519,7 @@ __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const 
     const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
     const T thrT = (T)io.thr;
     T *ft = io.FT + (size_t)slot * io.n + (size_t)k2 * io.n1;
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int u = 0; u < NB / 2; u++) {
         const int j = (threadIdx.x + u * NT) % LR;
@@ -524,17 +527,23 @@ __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const 
         for (int r = 0; r < RO; r++) {
             const int k1 = j + r * 256;
             const int k = k2 + io.n2 * k1;
-            bool c = false;
+            bool c = false, hit = false;
             T F = 0;
             if (k >= io.fr0 && k <= io.fr1) {
                 F = cnorm(v[u * RS + r]) * raa + cnorm(v[(u + NB / 2) * RS + r]) * rbb;
-                ft[k1] = F;
-                c = io.emit_cand && k > io.fr0 && k < io.fr1 - 1 && !(F < thrT);
+                hit = !(F < thrT);
+                if (!io.sparse || hit) ft[k1] = F;
+                c = io.emit_cand && k > io.fr0 && k < io.fr1 - 1 && hit;
+            }
+            if (io.sparse) {
+                // the wavefront's 64 lanes hold 64 consecutive k1 (LR is a
+                // multiple of 64): one word of flags per wavefront
+                const unsigned long long hm = __ballot(hit);
+                if (lane == 0) io.cbits[((size_t)slot * io.n + (size_t)k2 * io.n1 + k1) >> 6] = hm;
             }
             if (io.emit_cand) {
                 const unsigned long long m = __ballot(c);
                 if (m) {
-                    const int lane = threadIdx.x & 63;
                     const int leader = __ffsll((long long)m) - 1;
                     int bpos = 0;
                     if (lane == leader) bpos = atomicAdd(io.cand_cnt, __popcll(m));
@@ -836,6 +845,27 @@ __device__ __forceinline__ bool cond_global(const T *__restrict__ ft, const T *_
     return !(fi < (T)a.thr || fnorm_at(ft, mu, a, i - 1) > fi || fi < fnorm_at(ft, mu, a, i + 1));
 }
 
+// Sparse F (no FStat normalisation): F at bin k if it is >= thr (its flag
+// is set), else -1, which is below every F >= thr, so the MAX_ALG 3
+// comparisons of a bin >= thr with a neighbour below thr come out as they
+// do with the neighbour's value.
+template <typename T>
+__device__ __forceinline__ T fsparse_at(const T *__restrict__ ft, const unsigned long long *__restrict__ bits,
+                                        const TrigArgs &a, int k)
+{
+    const int k2 = k & (a.n2 - 1), k1 = k >> a.l2;
+    const size_t idx = (size_t)k2 * a.n1 + k1;
+    return ((bits[idx >> 6] >> (idx & 63)) & 1ull) ? ft[idx] : (T)-1;
+}
+
+template <typename T>
+__device__ __forceinline__ bool cond_sparse(const T *__restrict__ ft, const unsigned long long *__restrict__ bits,
+                                            const TrigArgs &a, int i)
+{
+    const T fi = fsparse_at(ft, bits, a, i);
+    return !(fi < (T)a.thr || fsparse_at(ft, bits, a, i - 1) > fi || fi < fsparse_at(ft, bits, a, i + 1));
+}
+
 // MAX_ALG 3 (jobcore.c:556-603) with the sequential skip rule: i is
 // accepted iff cond(i) holds and the number of consecutive cond-true
 // indices immediately before i (down to fr0+1) is even.
@@ -953,10 +983,14 @@ __global__ void __launch_bounds__(256) k_candidates(const T *__restrict__ FT, Tr
             gslot = ce.slot;
             i = ce.ic;
             const T *ft = FT + (size_t)(gslot - a.slot0) * a.n;
-            if (cond_global(ft, (const T *)nullptr, a, i)) {
+            const unsigned long long *bits = a.sparse ? a.cbits + (size_t)(gslot - a.slot0) * (a.n >> 6) : nullptr;
+            auto cond = [&](int ii) {
+                return a.sparse ? cond_sparse(ft, bits, a, ii) : cond_global(ft, (const T *)nullptr, a, ii);
+            };
+            if (cond(i)) {
                 int run = 0;
                 int ig = i - 1;
-                while (ig >= ilo && cond_global(ft, (const T *)nullptr, a, ig)) {
+                while (ig >= ilo && cond(ig)) {
                     run++;
                     ig--;
                 }
@@ -972,7 +1006,7 @@ __global__ void __launch_bounds__(256) k_candidates(const T *__restrict__ FT, Tr
                     if (!veto) {
                         emit = true;
                         fo = (float)f_;
-                        Fo = (float)fnorm_at(ft, (const T *)nullptr, a, i);
+                        Fo = (float)(a.sparse ? fsparse_at(ft, bits, a, i) : fnorm_at(ft, (const T *)nullptr, a, i));
                     }
                 }
             }
@@ -1204,7 +1238,8 @@ template <typename T, bool NTZ>
 static hipError_t launch_spin_pass2_z(const FftPlan &p, const SpinArgs &s, hipStream_t st)
 {
     IO_SpinP2<T, NTZ> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1,
-                         s.emit_cand, s.slot0, s.thr, s.cand, s.cand_cnt, s.cand_cap};
+                         s.emit_cand, s.slot0, s.thr, s.cand, s.cand_cnt, s.cand_cap, s.cbits,
+                         (p.f2 && p.e2 == 2 * p.n1) ? s.sparse : 0};
     if (p.f2 && p.e2 == 2 * p.n1) {
         const cx<T> *tw = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
         const dim3 grid(p.n2, s.nslots);

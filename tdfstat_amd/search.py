@@ -82,7 +82,7 @@ class Record:
 
 
 class Context:
-    def __init__(self, sett, dets, prec=L.TDF_PREC_FP32, batch=0):
+    def __init__(self, sett, dets, prec=L.TDF_PREC_FP32, batch=0, keep_fstat=True):
         self.lib = L.load()
         self.sett = sett
         self.prec = prec
@@ -92,6 +92,8 @@ class Context:
         h = C.c_void_p()
         L.check(self.lib.tdf_ctx_create(C.byref(self._g), prec, self.batch, C.byref(h)), "tdf_ctx_create")
         self.h = h
+        # F kept at every bin (ctx.fstat) unless a caller only needs triggers
+        self.keep_fstat(keep_fstat)
         for k, d in enumerate(dets):
             x = np.ascontiguousarray(d.xDat, dtype=np.float32)
             ssb = np.ascontiguousarray(d.DetSSB, dtype=np.float64)
@@ -290,6 +292,9 @@ class Context:
             if cnt.value:
                 out[name.value.decode()] = (ms.value, cnt.value)
         return out
+
+    def keep_fstat(self, on=True):
+        L.check(self.lib.tdf_keep_fstat(self.h, int(bool(on))), "tdf_keep_fstat")
 
     def synchronize(self):
         L.check(self.lib.tdf_synchronize(self.h), "tdf_synchronize")

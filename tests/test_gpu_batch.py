@@ -53,3 +53,31 @@ def test_batch_equals_per_point(prec, thr):
     _flush(f"  {len(recs)} records, {ntrig} triggers")
     if thr == 2.0:
         assert ntrig > (1 << 16)  # more than the pipelined read-back holds for some points
+
+
+@pytest.mark.parametrize("prec", [L.TDF_PREC_FP32, L.TDF_PREC_FP64])
+@pytest.mark.parametrize("thr", [20.0, 5.0])
+def test_sparse_f_gives_the_same_records(prec, thr):
+    """Without kept F arrays the fused pass 2 writes F only at bins >= thr
+    plus one flag bit per bin; the trigger rule then reads neighbours below
+    thr as below the bin.  Records and (f, F) pairs are those of the full-F
+    path, bit for bit."""
+    _flush(f"sparse F prec={prec} thr={thr}")
+    rng = np.random.default_rng(37)
+    seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=37,
+                             injections=[synth.random_injection(rng, 14.0) for _ in range(3)])
+    sett = seg.sett
+    opts = TS.default_opts(sett, thr=thr, sst=-6, spndr1=6, sstep=1)
+    pts = POINTS[:2] + POINTS[3:]
+    out = []
+    for keep in (True, False):
+        with TS.Context(sett, seg.dets, prec=prec, batch=8, keep_fstat=keep) as ctx:
+            plan = (512, 2048 if prec == L.TDF_PREC_FP64 else 4096, 1024, 1)
+            ctx.set_plan(*plan, 8)
+            out.append(ctx.job_core_batch(pts, opts, pair_cap=1 << 21))
+    (_, n0, r0), (_, n1, r1) = out
+    assert list(n0) == list(n1) and len(r0) == len(r1)
+    for a, b in zip(r0, r1):
+        assert (a.m, a.n, a.s) == (b.m, b.n, b.s)
+        np.testing.assert_array_equal(a.ffstat, b.ffstat)
+    _flush(f"  {sum(len(r.ffstat) for r in r0)} triggers")
