@@ -90,7 +90,10 @@ def kernel_bytes(sett, S, nslots, kname, prec64):
         # (xDatma, xDatmb complex + FP64 phase base per sample) once per launch
         return nslots * 2 * nfftf * c + nifo * N * (2 * c + 8)
     if kname == "spin_pass2_fft_fstat":
-        return nslots * (2 * nfftf * c + nF * r)
+        # read Fa, Fb; write one flag bit per bin below nfftf/2 (F itself only
+        # at the rare bins >= thr); with F kept at every bin (tdf_keep_fstat)
+        # it writes nF * r instead of the flags
+        return nslots * (2 * nfftf * c + nfftf // 16)
     if kname == "triggers":
         return nslots * nF * r
     return None
@@ -214,41 +217,31 @@ def cpu_baseline(cfg, seg, pts, budget_s=12.0):
                        f"{threads} threads, {el:.1f} s")
 
 
-def kernel_variant(kname, plan):
-    """(L, E, fused) of the spindown kernel kname under plan: the template
-    arguments that tell its compiled variants apart."""
-    n1, e1, e2, fused = plan["n1"], plan["e1"], plan["e2"], plan["fused"]
-    if kname == "spin_pass1_demod_fft":
-        return (None, e1, 0)
-    if kname == "spin_pass2_fft_fstat":
-        return (n1, e2, fused)
-    return None
+def plan_key(plan):
+    """The plan fields that tell the compiled spindown kernels apart:
+    n1, e1, e2, pass-2 form (0 LDS-tiled, 1 fused, 2 streaming), ntz."""
+    form = 2 if plan.get("stream") else int(plan["fused"])
+    return (plan["n1"], plan["e1"], plan["e2"], form, plan["ntz"])
 
 
 def measured_traffic(config, kname, plan, nslots):
     """HBM bytes per launch of kname from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (bytes per template of the same
-    kernel variant, scaled to nslots templates); (None, reason) when the
-    summary does not cover this variant."""
+    WRITE_SIZE summary of this workload (bytes per template measured under
+    the same plan, scaled to nslots templates); (None, reason) when the
+    summary was taken under another plan."""
     path = os.path.join(ROOT, "profiles", f"spindown_kernels_{config}_hbm_traffic.txt")
     if not os.path.exists(path):
         return None, "no counter summary for this workload"
-    want = kernel_variant(kname, plan)
+    want = plan_key(plan)
     with open(path) as f:
         for ln in f:
             if not ln.startswith("variant "):
                 continue
             kv = dict(t.split("=", 1) for t in ln.split()[1:])
-            if kv["kernel"] != kname:
-                continue
-            L_, E_, fu = int(kv["L"]), int(kv["E"]), int(kv["fused"])
-            if want[0] is None:  # pass 1: L = n2 = nfftf / n1
-                if L_ * plan["n1"] != int(kv["n"]) or E_ != want[1]:
-                    continue
-            elif (L_, E_, fu) != want:
-                continue
-            return int(float(kv["bytes_per_template"]) * nslots), os.path.relpath(path, ROOT)
-    return None, "variant not in the counter summary"
+            got = (int(kv["n1"]), int(kv["e1"]), int(kv["e2"]), int(kv["form"]), int(kv["ntz"]))
+            if kv["kernel"] == kname and got == want:
+                return int(float(kv["bytes_per_template"]) * nslots), os.path.relpath(path, ROOT)
+    return None, "counter summary taken under another plan"
 
 
 def main():
