@@ -9,6 +9,7 @@
  *   search/network/openmp/init.c:761-809      read_checkpoints
  *   search/network/openmp/init.c:510-705      set_search_range (range modes)
  *   search/network/openmp/auxi.c:63-90,264-332  dimless_to_code, gridr
+ *   search/network/openmp/common.c:22-33,155-553 veto lines of a band
  *
  * The trigger file has the reference's layout (format_version 2): root
  * attributes format_version, git_commit, t_start (t_end, totsgnl, walltime,
@@ -80,6 +81,33 @@ int tdf_dimless_to_code(const double M[16], const double pos_dimless[4], float p
  * (ignored, 1), s, mm, nn steps.  hemi 0: both hemispheres. */
 int tdf_set_search_range(const tdf_run_info *run, int gtype, int hemi, const float gcenter[4],
                          const float gsizes[4], const float gsteps[4], tdf_range *r);
+
+/* Veto lines (common.c:155-553).  lines[k] = {left, right} in radians of the
+ * band (0..pi), the first *numlines_band of them the narrowdown edges.
+ *   tdf_narrow_down_band: narrow_down_band -- lines[0..1] = the edges
+ *     [0, pi/2 - narrowdown] and [pi/2 + narrowdown, pi], *numlines_band = 2.
+ *   tdf_extract_band_vlines: extract_band_vlines -- reads
+ *     <indir>/lines/<det>lines*.csv for every detector (columns: spacing or
+ *     frequency [Hz], type 0 singlet / 1 comb of fixed width / 2 comb of
+ *     scaling width, offset [Hz], first and last harmonic, left and right
+ *     width [Hz]; '%' comments), widens every line in the band by the
+ *     demodulation broadening from detssb[d] (DetSSB, 3N doubles per
+ *     detector), appends them after the narrowdown lines (up to maxl in all),
+ *     writes band_vl_file (vlines_BBBB<label>.dat format, may be NULL) with
+ *     the veto fraction, and with run->veto_flag applies them
+ *     (*numlines_band = *nvlines_all_inband).
+ *   tdf_read_band_vlines: read_band_vlines -- the same lines from that file.
+ *   tdf_lines_veto_fraction: lines_veto_fraction -- the fraction of [0, pi]
+ *     covered by lines[lf..le) (sorted in place). */
+int tdf_narrow_down_band(double narrowdown, double lines[][2], int *numlines_band);
+int tdf_extract_band_vlines(const tdf_run_info *run, const double *const *detssb, double lines[][2], int maxl,
+                            int *numlines_band, int *nvlines_all_inband, const char *band_vl_file,
+                            float *veto_fraction);
+int tdf_read_band_vlines(const char *band_vl_file, double narrowdown, int veto_flag, double lines[][2], int maxl,
+                         int *numlines_band, int *nvlines_all_inband, float *veto_fraction);
+float tdf_lines_veto_fraction(double lines[][2], int lf, int le);
+/* get_fdot_range (common.c:22-33) */
+void tdf_fdot_range(double fpo, double dt, double *fdot_min, double *fdot_max);
 
 /* read_checkpoints (init.c:761-809): with checkp_flag and a readable state
  * file "pst mst nst sst FNum", resume there (an empty state file means the

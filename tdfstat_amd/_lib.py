@@ -161,6 +161,12 @@ SEARCH_SIGNATURES = {
     "tdf_read_checkpoint": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(TdfRange), _LLP]),
     "tdf_search": (C.c_int, [_P, C.POINTER(TdfRunInfo), C.POINTER(TdfRange), C.POINTER(TdfJobOpts),
                              C.c_char_p, _LLP, _LLP]),
+    "tdf_narrow_down_band": (C.c_int, [C.c_double, _DP, _IP]),
+    "tdf_extract_band_vlines": (C.c_int, [C.POINTER(TdfRunInfo), C.POINTER(_DP), _DP, C.c_int, _IP, _IP,
+                                          C.c_char_p, _FP]),
+    "tdf_read_band_vlines": (C.c_int, [C.c_char_p, C.c_double, C.c_int, _DP, C.c_int, _IP, _IP, _FP]),
+    "tdf_lines_veto_fraction": (C.c_float, [_DP, C.c_int, C.c_int]),
+    "tdf_fdot_range": (None, [C.c_double, C.c_double, _DP, _DP]),
 }
 
 _lib = None

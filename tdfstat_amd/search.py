@@ -359,6 +359,43 @@ def read_checkpoint(r, state_file=None, checkp_flag=0):
     return fnum.value
 
 
+MAXL = 512  # veto lines per band (struct.h:14)
+
+
+def band_vlines(ri, detssb, narrowdown=None, band_vl_file=None):
+    """Veto lines of the band (narrow_down_band + extract_band_vlines,
+    common.c:155-429): ri is the run (indir, det_name, fpo, B, N, dt,
+    veto_flag, narrowdown), detssb the per-detector DetSSB arrays [3N].
+    Returns (lines [k, 2] radians, numlines_band, nvlines_all_inband,
+    veto fraction)."""
+    lib = L.load_search()
+    lines = np.zeros((MAXL, 2), dtype=np.float64)
+    nb, nv = C.c_int(0), C.c_int(0)
+    nd = ri.narrowdown if narrowdown is None else narrowdown
+    if nd < 0.5 * math.pi:
+        L.check(lib.tdf_narrow_down_band(nd, L.dptr(lines), C.byref(nb)), "tdf_narrow_down_band")
+    arrs = [np.ascontiguousarray(d, dtype=np.float64) for d in detssb]
+    ptrs = (C.POINTER(C.c_double) * len(arrs))(*[L.dptr(a) for a in arrs])
+    vf = C.c_float(0)
+    L.check(lib.tdf_extract_band_vlines(C.byref(ri), ptrs, L.dptr(lines), MAXL, C.byref(nb), C.byref(nv),
+                                        band_vl_file.encode() if band_vl_file else None, C.byref(vf)),
+            "tdf_extract_band_vlines")
+    return lines[:nv.value].copy(), nb.value, nv.value, vf.value
+
+
+def read_band_vlines(path, narrowdown, veto_flag=0):
+    """read_band_vlines (common.c:432-513) after narrow_down_band."""
+    lib = L.load_search()
+    lines = np.zeros((MAXL, 2), dtype=np.float64)
+    nb, nv = C.c_int(0), C.c_int(0)
+    if narrowdown < 0.5 * math.pi:
+        L.check(lib.tdf_narrow_down_band(narrowdown, L.dptr(lines), C.byref(nb)), "tdf_narrow_down_band")
+    vf = C.c_float(0)
+    L.check(lib.tdf_read_band_vlines(path.encode(), narrowdown, veto_flag, L.dptr(lines), MAXL, C.byref(nb),
+                                     C.byref(nv), C.byref(vf)), "tdf_read_band_vlines")
+    return lines[:nv.value].copy(), nb.value, nv.value, vf.value
+
+
 def h5_read(path):
     """Records of a trigger file written by tdf_search / tdf_h5_extend."""
     lib = L.load_search()
