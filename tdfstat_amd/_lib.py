@@ -147,6 +147,22 @@ class TdfRunInfo(C.Structure):
     )
 
 
+class TdfCoincSegment(C.Structure):
+    _fields_ = [("seg", C.c_int), ("N", C.c_int), ("nfftf", C.c_int), ("dt", C.c_double),
+                ("narrowdown", C.c_double), ("nlines", C.c_int), ("lines", C.POINTER(C.c_double)),
+                ("nrec", C.c_longlong), ("recs", C.POINTER(TdfRecord)), ("pairs", C.POINTER(C.c_float))]
+
+
+class TdfCoincOpts(C.Structure):
+    _fields_ = [("refr", C.c_int), ("cthr", C.c_double), ("scalem", C.c_int), ("scalen", C.c_int),
+                ("scales", C.c_int), ("scalef", C.c_int), ("mincoin", C.c_int), ("shift", C.c_int)]
+
+
+class TdfCoincidence(C.Structure):
+    _fields_ = [("shift", C.c_char * 5), ("w", C.c_short), ("avg_snr", C.c_float), ("avg_f", C.c_float),
+                ("avg_fdot", C.c_float), ("avg_ra", C.c_float), ("avg_dec", C.c_float), ("first", C.c_longlong)]
+
+
 _LLP = C.POINTER(C.c_longlong)
 SEARCH_SIGNATURES = {
     "tdf_h5_init": (C.c_int, [C.c_char_p, C.POINTER(TdfRunInfo), C.POINTER(TdfRange)]),
@@ -167,6 +183,9 @@ SEARCH_SIGNATURES = {
     "tdf_read_band_vlines": (C.c_int, [C.c_char_p, C.c_double, C.c_int, _DP, C.c_int, _IP, _IP, _FP]),
     "tdf_lines_veto_fraction": (C.c_float, [_DP, C.c_int, C.c_int]),
     "tdf_fdot_range": (None, [C.c_double, C.c_double, _DP, _DP]),
+    "tdf_coincidences": (C.c_int, [C.POINTER(TdfCoincSegment), C.c_int, C.POINTER(TdfCoincOpts),
+                                   C.POINTER(TdfCoincidence), C.c_longlong, C.POINTER(C.c_short), _IP,
+                                   C.POINTER(C.c_short), C.c_longlong, _LLP, _LLP, _IP, _IP]),
 }
 
 _lib = None

@@ -1,0 +1,263 @@
+/* Coincidences between the trigger files of several time segments of one
+ * band (coincidences/coincidences.c:39-515): good candidates of every
+ * segment (select_goodcands, coincidences.c:430-515), grouping of the
+ * search grid points into coincidence cells (m, n, s scaled by
+ * scalem/n/s), per cell the loudest trigger of every segment in each
+ * frequency cell (f scaled by scalef), and a coincidence wherever at least
+ * mincoin segments meet; repeated for the 16 half-cell shifts in (m, n, s, f). */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "tdfstat_search.h"
+
+typedef struct {
+    int mc, nc, sc, orig;
+} tkey;
+
+static int cmp_tkey(const void *a, const void *b)
+{
+    const tkey *x = (const tkey *)a, *y = (const tkey *)b;
+    if (x->mc != y->mc) return (x->mc > y->mc) - (x->mc < y->mc);
+    if (x->nc != y->nc) return (x->nc > y->nc) - (x->nc < y->nc);
+    if (x->sc != y->sc) return (x->sc > y->sc) - (x->sc < y->sc);
+    return (x->orig > y->orig) - (x->orig < y->orig);  /* record order inside a cell */
+}
+
+typedef struct {
+    tdf_coincidence c;
+    int key;  /* position in the order of discovery (ties of the final sort) */
+} ckey;
+
+static int cmp_coinc(const void *a, const void *b)
+{
+    const ckey *x = (const ckey *)a, *y = (const ckey *)b;
+    if (y->c.w != x->c.w) return (int)y->c.w - (int)x->c.w;  /* more segments first */
+    if (y->c.avg_snr > x->c.avg_snr) return 1;               /* then the louder */
+    if (y->c.avg_snr < x->c.avg_snr) return -1;
+    return (x->key > y->key) - (x->key < y->key);
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+    const int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* good candidates of one segment as (f at the reference segment, F) pairs
+ * per record (select_goodcands) */
+static int good_candidates(const tdf_coinc_segment *sg, const tdf_coinc_opts *o, int *off, float *buf, long long cap,
+                           long long *used)
+{
+    long long n = 0;
+    for (long long j = 0; j < sg->nrec; j++) {
+        const tdf_record *r = &sg->recs[j];
+        off[j] = (int)n;
+        const float fdot_lin = (float)(r->fdot * M_PI * pow(sg->dt, 2));
+        for (int t = 0; t < r->ntrig; t++) {
+            float f = sg->pairs[2 * (r->offset + t)];
+            const float F = sg->pairs[2 * (r->offset + t) + 1];
+            if (F < o->cthr) continue;
+            if ((f <= M_PI_2 - sg->narrowdown) || (f >= M_PI_2 + sg->narrowdown)) continue;
+            int isline = 0;
+            for (int l = 0; l < sg->nlines; l++)
+                if (f >= sg->lines[2 * l] && f <= sg->lines[2 * l + 1]) {
+                    isline = 1;
+                    break;
+                }
+            if (isline) continue;
+            /* f moved to the reference segment by the spindown */
+            f = (float)(f + 2. * fdot_lin * (sg->N) * (o->refr - sg->seg));
+            if ((f < 0) || (f > M_PI)) continue;
+            if (n >= cap) return TDF_ERR_CAPACITY;
+            buf[2 * n] = f;
+            buf[2 * n + 1] = F;
+            n++;
+        }
+    }
+    off[sg->nrec] = (int)n;
+    *used = n;
+    return TDF_OK;
+}
+
+int tdf_coincidences(const tdf_coinc_segment *segs, int nseg, const tdf_coinc_opts *o, tdf_coincidence *out,
+                     long long out_cap, short *cseg, int *trig_mns, short *n_ccell_trigs, long long member_cap,
+                     long long *ncoin, long long *nmember, int *good, int *unique)
+{
+    if (!segs || nseg < 1 || nseg > 32767 || !o || !ncoin || !nmember || o->scalem < 1 || o->scalen < 1 ||
+        o->scales < 1 || o->scalef < 1 || o->shift < -1 || o->shift > 15)
+        return TDF_ERR_ARG;
+    const long long nrec = segs[0].nrec;
+    for (int s = 1; s < nseg; s++)
+        if (segs[s].nrec != nrec) return TDF_ERR_ARG;  /* one search grid: records line up by position */
+    const int nfftf = segs[nseg - 1].nfftf;
+    const int nfc = nfftf / o->scalef;  /* frequency cells */
+    int rc = TDF_OK;
+    /* good candidates of every segment */
+    int **off = (int **)calloc((size_t)nseg, sizeof(int *));
+    float **gp = (float **)calloc((size_t)nseg, sizeof(float *));
+    if (!off || !gp) rc = TDF_ERR_NOMEM;
+    for (int s = 0; s < nseg && rc == TDF_OK; s++) {
+        long long tot = 0;
+        for (long long j = 0; j < nrec; j++) tot += segs[s].recs[j].ntrig;
+        off[s] = (int *)malloc(sizeof(int) * (size_t)(nrec + 1));
+        gp[s] = (float *)malloc(sizeof(float) * 2 * (size_t)(tot > 0 ? tot : 1));
+        if (!off[s] || !gp[s]) {
+            rc = TDF_ERR_NOMEM;
+            break;
+        }
+        long long used = 0;
+        rc = good_candidates(&segs[s], o, off[s], gp[s], tot, &used);
+        if (good) good[s] = (int)used;
+    }
+    tkey *keys = (tkey *)malloc(sizeof(tkey) * (size_t)(nrec > 0 ? nrec : 1));
+    int *best = (int *)malloc(sizeof(int) * (size_t)nseg * nfc);
+    float *bsnr = (float *)calloc((size_t)nseg * nfc, sizeof(float));
+    float *bf = (float *)calloc((size_t)nseg * nfc, sizeof(float));
+    short *ntc = (short *)calloc((size_t)nseg * nfc, sizeof(short));
+    int *dirty = (int *)malloc(sizeof(int) * (size_t)nseg * nfc);
+    int *fics = (int *)malloc(sizeof(int) * (size_t)nseg * nfc);
+    char *seen = (char *)calloc((size_t)nfc, 1);
+    ckey *found = NULL;
+    long long nfound = 0, fcap = 0;
+    if (rc == TDF_OK && (!keys || !best || !bsnr || !bf || !ntc || !dirty || !fics || !seen)) rc = TDF_ERR_NOMEM;
+    if (rc == TDF_OK)
+        for (long long i = 0; i < (long long)nseg * nfc; i++) best[i] = -1;
+    long long nc_out = 0, nm_out = 0;
+    const int ish0 = o->shift >= 0 ? o->shift : 0, ish1 = o->shift >= 0 ? o->shift : 15;
+    for (int ish = ish0; ish <= ish1 && rc == TDF_OK; ish++) {
+        const int sh[4] = {ish & 1, (ish >> 1) & 1, (ish >> 2) & 1, (ish >> 3) & 1};  /* f, s, n, m */
+        const float shm = sh[3] * 0.5f, shn = sh[2] * 0.5f, shs = sh[1] * 0.5f, shf = sh[0] * 0.5f;
+        for (long long j = 0; j < nrec; j++) {
+            const tdf_record *r = &segs[0].recs[j];
+            keys[j].mc = (int)floorf(r->m / o->scalem - shm);
+            keys[j].nc = (int)floorf(r->n / o->scalen - shn);
+            keys[j].sc = (int)floorf(r->s / o->scales - shs);
+            keys[j].orig = (int)j;
+        }
+        qsort(keys, (size_t)nrec, sizeof(tkey), cmp_tkey);
+        if (unique)
+            for (int s = 0; s < nseg; s++) unique[(ish - ish0) * nseg + s] = 0;
+        nfound = 0;
+        for (long long j0 = 0; j0 < nrec && rc == TDF_OK;) {
+            long long j1 = j0 + 1;
+            while (j1 < nrec && keys[j1].mc == keys[j0].mc && keys[j1].nc == keys[j0].nc &&
+                   keys[j1].sc == keys[j0].sc)
+                j1++;
+            /* pass 1: the loudest pair of every (segment, frequency cell) */
+            int nd = 0, nf = 0;
+            for (long long jj = j0; jj < j1; jj++) {
+                const int cidx = keys[jj].orig;
+                for (int s = 0; s < nseg; s++) {
+                    for (int k = off[s][cidx]; k < off[s][cidx + 1]; k++) {
+                        const float f = gp[s][2 * k];
+                        const float snr2 = (float)(2. * gp[s][2 * k + 1] - 4.);
+                        const int fic = (int)floorf((float)(f / (M_PI / nfftf) / o->scalef - shf));
+                        if ((unsigned)fic >= (unsigned)nfc) continue;
+                        const int idx = s * nfc + fic;
+                        if (ntc[idx] == 0) {  /* first touch */
+                            dirty[nd++] = idx;
+                            if (!seen[fic]) {
+                                seen[fic] = 1;
+                                fics[nf++] = fic;
+                            }
+                        }
+                        ntc[idx]++;
+                        if (snr2 > bsnr[idx]) {
+                            bsnr[idx] = snr2;
+                            best[idx] = cidx;
+                            bf[idx] = f;
+                        }
+                    }
+                }
+            }
+            /* pass 2: frequency cells in ascending order (cells nobody
+             * touched hold no coincidence) */
+            qsort(fics, (size_t)nf, sizeof(int), cmp_int);
+            for (int q = 0; q < nf && rc == TDF_OK; q++) {
+                const int fic = fics[q];
+                int w = 0;
+                float ssnr = 0.f, sf = 0.f, sfd = 0.f, sra = 0.f, sdec = 0.f;
+                for (int s = 0; s < nseg; s++) {
+                    const int idx = s * nfc + fic;
+                    if (best[idx] < 0) continue;
+                    const tdf_record *r = &segs[0].recs[best[idx]];
+                    if (unique) unique[(ish - ish0) * nseg + s]++;
+                    ssnr += bsnr[idx];
+                    sf += bf[idx];
+                    sfd += r->fdot;
+                    sra += r->ra;
+                    sdec += r->dec;
+                    w++;
+                }
+                if (w < o->mincoin) continue;
+                if (nfound == fcap) {
+                    fcap = fcap ? 2 * fcap : 1024;
+                    ckey *nfd = (ckey *)realloc(found, sizeof(ckey) * (size_t)fcap);
+                    if (!nfd) {
+                        rc = TDF_ERR_NOMEM;
+                        break;
+                    }
+                    found = nfd;
+                }
+                if (nm_out + w > member_cap) {
+                    rc = TDF_ERR_CAPACITY;
+                    break;
+                }
+                tdf_coincidence *c = &found[nfound].c;
+                found[nfound].key = (int)nfound;
+                c->w = (short)w;
+                for (int b = 0; b < 4; b++) c->shift[b] = (char)('0' + sh[3 - b]);
+                c->shift[4] = '\0';
+                c->avg_snr = sqrtf(ssnr / w);
+                c->avg_f = sf / w;
+                c->avg_fdot = sfd / w;
+                c->avg_ra = sra / w;
+                c->avg_dec = sdec / w;
+                c->first = nm_out;
+                for (int s = 0; s < nseg; s++) {
+                    const int idx = s * nfc + fic;
+                    if (best[idx] < 0) continue;
+                    if (cseg) cseg[nm_out] = (short)s;
+                    if (trig_mns) trig_mns[nm_out] = best[idx];
+                    if (n_ccell_trigs) n_ccell_trigs[nm_out] = ntc[idx];
+                    nm_out++;
+                }
+                nfound++;
+            }
+            for (int q = 0; q < nd; q++) {
+                const int idx = dirty[q];
+                best[idx] = -1;
+                bsnr[idx] = 0.f;
+                bf[idx] = 0.f;
+                ntc[idx] = 0;
+            }
+            for (int q = 0; q < nf; q++) seen[fics[q]] = 0;
+            j0 = j1;
+        }
+        /* this shift's coincidences: more segments first, then louder */
+        if (rc == TDF_OK) {
+            qsort(found, (size_t)nfound, sizeof(ckey), cmp_coinc);
+            if (nc_out + nfound > out_cap) rc = TDF_ERR_CAPACITY;
+            for (long long i = 0; i < nfound && rc == TDF_OK; i++) out[nc_out++] = found[i].c;
+        }
+    }
+    *ncoin = nc_out;
+    *nmember = nm_out;
+    for (int s = 0; s < nseg; s++) {
+        if (off) free(off[s]);
+        if (gp) free(gp[s]);
+    }
+    free(off);
+    free(gp);
+    free(keys);
+    free(best);
+    free(bsnr);
+    free(bf);
+    free(ntc);
+    free(dirty);
+    free(fics);
+    free(seen);
+    free(found);
+    return rc;
+}

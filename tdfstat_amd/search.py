@@ -396,6 +396,68 @@ def read_band_vlines(path, narrowdown, veto_flag=0):
     return lines[:nv.value].copy(), nb.value, nv.value, vf.value
 
 
+def _c_records(records):
+    """(TdfRecord array, pairs float32) of a list of Record."""
+    n = len(records)
+    recs = (L.TdfRecord * max(1, n))()
+    chunks, off = [], 0
+    for k, r in enumerate(records):
+        ff = np.asarray(r.ffstat, dtype=np.float32).reshape(-1, 2)
+        recs[k].m, recs[k].n, recs[k].s = r.m, r.n, r.s
+        recs[k].ra, recs[k].dec, recs[k].fdot = r.ra, r.dec, r.fdot
+        recs[k].ntrig, recs[k].offset = len(ff), off
+        chunks.append(ff)
+        off += len(ff)
+    pairs = np.ascontiguousarray(np.concatenate(chunks).reshape(-1)) if off else np.zeros(2, np.float32)
+    return recs, pairs
+
+
+def coincidences(segments, refr, cthr, scalem=1, scalen=1, scales=4, scalef=32, mincoin=3, shift=-1):
+    """Coincidences between segments (coincidences/coincidences.c:39-515)
+    through libtdfstat_search.so.  segments: dicts with seg, N, nfftf, dt,
+    narrowdown, lines ([k, 2] radians) and records (Record list, the same
+    grid in every segment).  Returns (coincidences, good, unique): every
+    coincidence a dict with shift ("mnsf"), w, avg_snr, avg_f, avg_fdot,
+    avg_ra, avg_dec and members [(segment index, record index, triggers in
+    the cell)]."""
+    lib = L.load_search()
+    nseg = len(segments)
+    keep, segs = [], (L.TdfCoincSegment * nseg)()
+    total = 0
+    for k, sg in enumerate(segments):
+        recs, pairs = _c_records(sg["records"])
+        lines = np.ascontiguousarray(np.asarray(sg.get("lines", np.zeros((0, 2))), dtype=np.float64).reshape(-1, 2))
+        keep += [recs, pairs, lines]
+        segs[k].seg, segs[k].N, segs[k].nfftf = sg["seg"], sg["N"], sg["nfftf"]
+        segs[k].dt, segs[k].narrowdown = sg["dt"], sg["narrowdown"]
+        segs[k].nlines = len(lines)
+        segs[k].lines = L.dptr(lines) if len(lines) else C.POINTER(C.c_double)()
+        segs[k].nrec = len(sg["records"])
+        segs[k].recs = recs
+        segs[k].pairs = L.fptr(pairs)
+        total += int(pairs.size // 2)
+    o = L.TdfCoincOpts(refr, cthr, scalem, scalen, scales, scalef, mincoin, shift)
+    nsh = 16 if shift < 0 else 1
+    cap = max(16, total * nsh)
+    out = (L.TdfCoincidence * cap)()
+    cseg = np.zeros(cap * nseg, dtype=np.int16)
+    tm = np.zeros(cap * nseg, dtype=np.int32)
+    ncc = np.zeros(cap * nseg, dtype=np.int16)
+    nc, nm = C.c_longlong(0), C.c_longlong(0)
+    good = np.zeros(nseg, dtype=np.int32)
+    uniq = np.zeros(nsh * nseg, dtype=np.int32)
+    L.check(lib.tdf_coincidences(segs, nseg, C.byref(o), out, cap, cseg.ctypes.data_as(C.POINTER(C.c_short)),
+                                 L.iptr(tm), ncc.ctypes.data_as(C.POINTER(C.c_short)), cap * nseg, C.byref(nc),
+                                 C.byref(nm), L.iptr(good), L.iptr(uniq)), "tdf_coincidences")
+    res = []
+    for k in range(nc.value):
+        c = out[k]
+        mem = [(int(cseg[c.first + i]), int(tm[c.first + i]), int(ncc[c.first + i])) for i in range(c.w)]
+        res.append(dict(shift=c.shift.decode(), w=c.w, avg_snr=c.avg_snr, avg_f=c.avg_f, avg_fdot=c.avg_fdot,
+                        avg_ra=c.avg_ra, avg_dec=c.avg_dec, members=mem))
+    return res, good, uniq.reshape(nsh, nseg)
+
+
 def h5_read(path):
     """Records of a trigger file written by tdf_search / tdf_h5_extend."""
     lib = L.load_search()

@@ -1,0 +1,95 @@
+"""CPU: coincidences between segments (coincidences/coincidences.c) through
+libtdfstat_search.so, against known answers and the Python restatement in
+oracle/coinc_oracle.py."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import coinc_oracle as CO
+from tdfstat_amd import search as TS
+
+NFFTF = 1 << 14
+
+
+def _grid(nm=3, nn=3, ns=8):
+    return [(float(m), float(n), float(s)) for m in range(-nm, nm + 1) for n in range(-nn, nn + 1)
+            for s in range(-ns, 1)]
+
+
+def _segments(rng, nseg=4, noise=3, signal=None, grid=None):
+    """nseg segments over one grid; every record gets `noise` random
+    triggers; signal = (record index, f at the reference segment, F) adds one
+    trigger per segment at the frequency that moves to f at refr."""
+    grid = grid or _grid()
+    segs = []
+    for k in range(nseg):
+        recs = []
+        for j, (m, n, s) in enumerate(grid):
+            fdot = -2e-10 * (1 + 0.1 * s)
+            ff = [(rng.uniform(0.3, math.pi - 0.3), rng.uniform(10, 40)) for _ in range(noise)]
+            if signal and j == signal[0]:
+                fdot_lin = float(np.float32(fdot * math.pi * 2.0 ** 2))
+                f0 = signal[1] - 2.0 * fdot_lin * 86164 * (5 - (k + 1))
+                ff.append((f0, signal[2]))
+            recs.append(TS.Record(m, n, s, 0.1 * m, 0.05 * n, fdot, np.array(ff, dtype=np.float32)))
+        segs.append(dict(seg=k + 1, N=86164, nfftf=NFFTF, dt=2.0, narrowdown=0.95 * math.pi / 2,
+                         lines=np.array([[1.0, 1.01]]), records=recs))
+    return segs
+
+
+def test_signal_in_every_segment_is_the_top_coincidence():
+    """A trigger at one grid point in all 6 segments whose frequencies
+    coincide at the reference segment is found with w = 6 in every shift,
+    at the average of its members, and ranks first."""
+    rng = np.random.default_rng(3)
+    f_ref = 1.7
+    segs = _segments(rng, nseg=6, noise=1, signal=(40, f_ref, 60.0))
+    res, good, uniq = TS.coincidences(segs, refr=5, cthr=15.0, scalef=8, mincoin=3)
+    for sh in range(16):
+        top = [c for c in res if c["shift"] == format(sh, "04b")][0]
+        assert top["w"] == 6 and {m[1] for m in top["members"]} == {40}
+        assert abs(top["avg_f"] - f_ref) < 1e-5
+        assert top["avg_snr"] == pytest.approx(math.sqrt(2 * 60.0 - 4), rel=1e-6)
+    assert len(good) == 6 and all(g > 0 for g in good)
+
+
+def test_matches_oracle_on_random_triggers():
+    rng = np.random.default_rng(11)
+    segs = _segments(rng, nseg=5, noise=4)
+    for kw in (dict(scalef=64, scales=4, mincoin=2), dict(scalef=16, scalem=2, scalen=2, scales=2, mincoin=3)):
+        got, good, uniq = TS.coincidences(segs, refr=3, cthr=18.0, **kw)
+        want, wgood, wuniq = CO.coincidences(segs, refr=3, cthr=18.0, **kw)
+        assert list(good) == wgood and uniq.tolist() == wuniq
+        assert len(got) == len(want) > 0
+        for g, w in zip(got, want):
+            assert g["shift"] == w["shift"] and g["w"] == w["w"] and g["members"] == w["members"]
+            for k in ("avg_snr", "avg_f", "avg_fdot", "avg_ra", "avg_dec"):
+                assert g[k] == pytest.approx(w[k], rel=1e-6, abs=1e-12), k
+
+
+def test_selection_rules():
+    """Threshold, narrowdown edges, veto lines and the band edge after the
+    spindown shift each remove a candidate (select_goodcands)."""
+    grid = [(0.0, 0.0, 0.0)]
+    def seg(k, ff, fdot=0.0):
+        r = TS.Record(0.0, 0.0, 0.0, 0.0, 0.0, fdot, np.array(ff, dtype=np.float32))
+        return dict(seg=k, N=1000, nfftf=NFFTF, dt=2.0, narrowdown=1.0, lines=np.array([[1.2, 1.3]]),
+                    records=[r])
+    ff = [(1.5, 20.0), (1.5, 14.0), (0.5, 30.0), (1.25, 30.0), (2.6, 30.0), (2.5, 30.0)]
+    _, good, _ = TS.coincidences([seg(1, ff)], refr=1, cthr=15.0, mincoin=1)
+    # kept: (1.5, 20) and (2.5, 30): 14 < cthr, 0.5 and 2.6 outside pi/2 -+ 1.0, 1.25 in a line
+    assert list(good) == [2]
+    # a spindown that moves f out of [0, pi] at the reference segment drops it
+    _, good, _ = TS.coincidences([seg(1, [(2.5, 30.0)], fdot=-1e-4)], refr=1000, cthr=15.0, mincoin=1)
+    assert list(good) == [0]
+
+
+def test_single_shift_and_mincoin():
+    rng = np.random.default_rng(5)
+    segs = _segments(rng, nseg=4, noise=2, signal=(7, 2.2, 50.0))
+    res, _, uniq = TS.coincidences(segs, refr=5, cthr=15.0, scalef=32, mincoin=4, shift=0b0101)
+    assert uniq.shape == (1, 4)
+    assert res and all(c["shift"] == "0101" and c["w"] >= 4 for c in res)
+    want, _, _ = CO.coincidences(segs, refr=5, cthr=15.0, scalef=32, mincoin=4, shift=0b0101)
+    assert [c["members"] for c in res] == [c["members"] for c in want]
