@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--per-step", type=int, default=0,
                     help="sky points per step (one tdf_job_core_batch call); 1 = one tdf_job_core call per step; "
                          "0 = the workload's default")
+    ap.add_argument("--no-ceilings", action="store_true", help="skip the plain read/write/copy ceiling probe")
+    ap.add_argument("--ceilings-only", action="store_true", help="print the ceiling probe's JSON line and exit")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -278,6 +280,16 @@ def main():
         s_ = src.sum()          # 256 MiB read (reduction kernel)
         torch.cuda.synchronize()
         del src, dst
+
+    ceil = None
+    if not args.no_ceilings or args.ceilings_only:
+        from tdfstat_amd import search as TS
+        ceil = TS.memory_ceilings(2 << 30, 10)
+        print(f"[bench] memory ceilings (GB/s, 2 GiB buffers): read {ceil['read']:.0f} write {ceil['write']:.0f} "
+              f"copy {ceil['copy']:.0f}", file=sys.stderr, flush=True)
+        if args.ceilings_only:
+            print(json.dumps({"memory_ceilings": ceil}), flush=True)
+            return
 
     cfg, seg = make_workload(args.config)
     sett = seg.sett
@@ -318,6 +330,15 @@ def main():
     traffic, tsrc = measured_traffic(args.config, dom, line_plan, nslots_avg)
     bt = algorithmic_bytes(sett, S, False)
     pipe = value * bt / 1e9
+    # each pass against 8 TB/s and against the plain-kernel ceiling of its
+    # access kind: pass 1 is a write stream, pass 2 a read stream
+    passes = {}
+    for k, kind in (("spin_pass1_demod_fft", "write"), ("spin_pass2_fft_fstat", "read")):
+        if k not in kt:
+            continue
+        a_ = kernel_bytes(sett, S, nslots_avg, k, False) / (kt[k][0] / kt[k][1] * 1e-3) / 1e9
+        passes[k] = {"achieved": round(a_, 1), "frac": round(a_ / HBM_PEAK_GBS, 4), "ceiling_kind": kind,
+                     "frac_of_ceiling": round(a_ / ceil[kind], 4) if ceil else None}
     line = {
         "metric": "templates/s (sky x spindown), FP32 bulk + FP64 phase",
         "value": round(value, 1),
@@ -342,6 +363,8 @@ def main():
                      "bytes_per_launch": int(kb), "traffic_source": tsrc},
         "pipeline_roofline": {"bytes_per_template": int(bt), "achieved": round(pipe, 1),
                               "frac": round(pipe / HBM_PEAK_GBS, 4)},
+        "passes": passes,
+        "memory_ceilings": ceil,
         "fp64_path": fp64,
         "fp32_vs_fp64": fperr,
         "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},

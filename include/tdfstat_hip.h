@@ -275,6 +275,16 @@ int tdf_kernel_times(tdf_ctx *ctx, int slot, double *ms, long long *launches,
 int tdf_keep_fstat(tdf_ctx *ctx, int on);
 int tdf_synchronize(tdf_ctx *ctx);
 
+/* ---- memory-system ceilings ---- */
+/* Times plain grid-stride kernels with 16-byte lanes on the current device
+ * over buffers of `bytes` each (two are allocated and freed by the call;
+ * bytes a multiple of 16), `reps` launches each after two warm-ups:
+ * gbps = {read, read non-temporal, write, write non-temporal, copy, copy
+ * non-temporal} in GB/s, a copy counting its read and its write bytes.
+ * These are the practical ceilings of the write-bound spindown pass 1 and
+ * the read-bound pass 2.  No reference counterpart. */
+int tdf_memory_ceilings(size_t bytes, int reps, double gbps[6]);
+
 #ifdef __cplusplus
 }
 #endif

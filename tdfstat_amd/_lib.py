@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtdfstat_hip.so")
 
 TDF_OK = 0
+TDF_ERR_ARG = -1
 TDF_PREC_FP32 = 0
 TDF_PREC_FP64 = 1
 
@@ -116,6 +117,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_char_p)]),
     "tdf_keep_fstat": (C.c_int, [_P, C.c_int]),
     "tdf_synchronize": (C.c_int, [_P]),
+    "tdf_memory_ceilings": (C.c_int, [C.c_size_t, C.c_int, _DP]),
 }
 
 # libtdfstat_search.so (include/tdfstat_search.h): the C host driver

@@ -507,18 +507,27 @@ __device__ __forceinline__ void pass2_load(const IO_SpinP2<T, NTZ> &io, int slot
 
 // LO: the band ends below n/2 (fr1 < n/2), so only k1 < L/2 (r < RS/2)
 // can hold bins of the band and the last stage forms only those outputs.
-template <typename T, int L, bool NTZ, bool LO>
+// raa, rbb: 1/aa and 1/bb of the launch's sky point (F = |Fa|^2/aa + |Fb|^2/bb).
+// after_first() runs once the first stage's barrier has passed: every
+// wavefront has then consumed its inputs v and written the tile.
+struct no_op {
+    __device__ void operator()() const {}
+};
+template <typename T, int L, bool NTZ, bool LO, class AF = no_op>
 __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const cx<T> *__restrict__ twL,
-                                             cx<T> *tile, int slot, int k2, cx<T> *v)
+                                             cx<T> *tile, int slot, int k2, cx<T> *v, T raa, T rbb,
+                                             AF after_first = AF())
 {
     constexpr int NCOL = 2, NT = L / 8, RS = L / 256, NB = 16 / RS, LR = L / RS;
     constexpr int RO = LO ? RS / 2 : RS;  // outputs per butterfly that are used
     stockham_first_regs<T, -1, L, NCOL, true>(tile, v);
+    after_first();
     stockham_stage<T, -1, L, NCOL, 16, 16>(tile, twL);
     stockham_last_regs<T, -1, L, NCOL, RS, 256, LO>(tile, twL, v);
-    const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
     const T thrT = (T)io.thr;
     T *ft = io.FT + (size_t)slot * io.n + (size_t)k2 * io.n1;
+    // flag words of the column (its first bin is a multiple of 64: n1 >= 512)
+    unsigned long long *cbw = io.sparse ? io.cbits + (((size_t)slot * io.n + (size_t)k2 * io.n1) >> 6) : nullptr;
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int u = 0; u < NB / 2; u++) {
@@ -539,7 +548,7 @@ __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const 
                 // the wavefront's 64 lanes hold 64 consecutive k1 (LR is a
                 // multiple of 64): one word of flags per wavefront
                 const unsigned long long hm = __ballot(hit);
-                if (lane == 0) io.cbits[((size_t)slot * io.n + (size_t)k2 * io.n1 + k1) >> 6] = hm;
+                if (lane == 0) cbw[k1 >> 6] = hm;
             }
             if (io.emit_cand) {
                 const unsigned long long m = __ballot(c);
@@ -565,15 +574,16 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io
     __shared__ cx<T> tile[2 * ldsgeom<L>::LP];
     cx<T> v[16];
     pass2_load<T, L, NTZ>(io, blockIdx.y, blockIdx.x, v);
-    pass2_column<T, L, NTZ, LO>(io, twL, tile, blockIdx.y, blockIdx.x, v);
+    pass2_column<T, L, NTZ, LO>(io, twL, tile, blockIdx.y, blockIdx.x, v, (T)1 / (T)io.sky[0], (T)1 / (T)io.sky[1]);
 }
 
 // Streaming form (FP32): a grid of resident workgroups walks the columns
 // item = k2 + n2*slot (stride gridDim.x).  A column's Z (2*L complex,
 // rows q with a, b interleaved) is copied by the memory system straight
 // into an LDS staging area (buffer loads to LDS, 1 KiB per wavefront
-// instruction); the copy of the next column is issued as soon as the
-// current one has been read into registers, so it runs while the column is
+// instruction); the copy of the next column is issued right after the
+// first stage's barrier (by then every wavefront has read the current
+// column from the staging area), so it runs while the column is
 // transformed.
 template <int L>
 struct stream_geom {
@@ -589,7 +599,9 @@ __device__ __forceinline__ void stream_issue(const IO_SpinP2<T, NTZ> &io, int it
     const int slot = item / io.n2, k2 = item % io.n2;
     const cx<T> *zc = io.Z + (size_t)slot * 2 * io.n + (size_t)k2 * io.n1 * 2;
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(zc, (size_t)2 * L * sizeof(cx<T>));
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wavefront index, known uniform to the compiler: the chunk offset
+    // is then a scalar operand (no per-lane loop around each copy)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
 #pragma unroll
     for (int i = 0; i < G::PER_WAVE; i++) {
         const int m = w * G::PER_WAVE + i;  // chunk: rows 64m .. 64m+63
@@ -608,6 +620,7 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_stream(IO_SpinP2<T, NTZ> i
     int item = blockIdx.x;
     if (item < nitems) stream_issue<T, L, NTZ>(io, item, raw);
     const int c = threadIdx.x & 1, j = threadIdx.x >> 1;
+    const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
     for (; item < nitems; item += gridDim.x) {
         // this wavefront's copies (and earlier stores) are done; the barrier
         // makes every wavefront's part visible
@@ -616,10 +629,11 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_stream(IO_SpinP2<T, NTZ> i
         cx<T> v[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) v[r] = raw[2 * (j + r * (L / 16)) + c];
-        __syncthreads();  // the staging area is read: the next copy may land
         const int nxt = item + gridDim.x;
-        if (nxt < nitems) stream_issue<T, L, NTZ>(io, nxt, raw);
-        pass2_column<T, L, NTZ, LO>(io, twL, tile, item / io.n2, item % io.n2, v);
+        pass2_column<T, L, NTZ, LO>(io, twL, tile, item / io.n2, item % io.n2, v, raa, rbb, [&]() {
+            // the staging area is read: the next copy may land
+            if (nxt < nitems) stream_issue<T, L, NTZ>(io, nxt, raw);
+        });
     }
 }
 

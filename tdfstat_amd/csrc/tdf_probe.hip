@@ -1,0 +1,106 @@
+// Memory-system ceilings: plain grid-stride kernels with 16-byte lanes that
+// only read, only write or copy, timed with HIP events.  The spindown passes
+// are a write stream (pass 1 stores Fa, Fb) and a read stream (pass 2 loads
+// them), so these rates, not the 8 TB/s pin rate of HBM3E, are the practical
+// ceilings each pass is compared with.  No counterpart in the reference.
+#include <hip/hip_runtime.h>
+
+#include "../../include/tdfstat_hip.h"
+
+namespace tdf {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__global__ void __launch_bounds__(256) k_probe_read(const f4 *__restrict__ a, size_t n, float *sink)
+{
+    f4 s = {0.f, 0.f, 0.f, 0.f};
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const f4 v = NT ? __builtin_nontemporal_load(&a[i]) : a[i];
+        s += v;
+    }
+    // a data-dependent store the compiler cannot remove (the buffer holds
+    // zeros, so it never happens)
+    if (s.x + s.y + s.z + s.w == 1234.5f) sink[0] = s.x;
+}
+
+template <bool NT>
+__global__ void __launch_bounds__(256) k_probe_write(f4 *__restrict__ a, size_t n)
+{
+    const f4 v = {0.f, 0.f, 0.f, 0.f};
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        if (NT) __builtin_nontemporal_store(v, &a[i]);
+        else a[i] = v;
+    }
+}
+
+template <bool NT>
+__global__ void __launch_bounds__(256) k_probe_copy(const f4 *__restrict__ a, f4 *__restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        if (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &b[i]);
+        else b[i] = a[i];
+    }
+}
+
+}  // namespace
+}  // namespace tdf
+
+using namespace tdf;
+
+extern "C" int tdf_memory_ceilings(size_t bytes, int reps, double gbps[6])
+{
+    if (!gbps || bytes < 16 || bytes % 16 || reps < 1) return TDF_ERR_ARG;
+    const size_t n = bytes / 16;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TDF_ERR_HIP;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return TDF_ERR_HIP;
+    const unsigned grid = (unsigned)ncu * 32;  // 32 workgroups of 256 per CU
+    f4 *a = nullptr, *b = nullptr;
+    float *sink = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TDF_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipMalloc(&sink, sizeof(float)) != hipSuccess) {
+        rc = TDF_ERR_NOMEM;
+        goto done;
+    }
+    if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 0, bytes, st) != hipSuccess ||
+        hipMemsetAsync(b, 0, bytes, st) != hipSuccess) {
+        rc = TDF_ERR_HIP;
+        goto done;
+    }
+    for (int k = 0; k < 6 && rc == TDF_OK; k++) {
+        auto launch = [&]() {
+            switch (k) {
+            case 0: hipLaunchKernelGGL(k_probe_read<false>, dim3(grid), dim3(256), 0, st, a, n, sink); break;
+            case 1: hipLaunchKernelGGL(k_probe_read<true>, dim3(grid), dim3(256), 0, st, a, n, sink); break;
+            case 2: hipLaunchKernelGGL(k_probe_write<false>, dim3(grid), dim3(256), 0, st, b, n); break;
+            case 3: hipLaunchKernelGGL(k_probe_write<true>, dim3(grid), dim3(256), 0, st, b, n); break;
+            case 4: hipLaunchKernelGGL(k_probe_copy<false>, dim3(grid), dim3(256), 0, st, a, b, n); break;
+            default: hipLaunchKernelGGL(k_probe_copy<true>, dim3(grid), dim3(256), 0, st, a, b, n); break;
+            }
+        };
+        launch();  // warm-up
+        launch();
+        if (hipEventRecord(e0, st) != hipSuccess) rc = TDF_ERR_HIP;
+        for (int r = 0; r < reps && rc == TDF_OK; r++) launch();
+        float ms = 0.f;
+        if (rc == TDF_OK && (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+                             hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = TDF_ERR_HIP;
+        if (rc == TDF_OK) gbps[k] = (k >= 4 ? 2.0 : 1.0) * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    }
+done:
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(sink);
+    return rc;
+}

@@ -300,6 +300,18 @@ class Context:
         L.check(self.lib.tdf_synchronize(self.h), "tdf_synchronize")
 
 
+def memory_ceilings(nbytes=2 << 30, reps=10):
+    """Plain read / write / copy rates of the current GPU in GB/s (the best
+    of ordinary and non-temporal accesses; a copy counts read + write bytes)
+    over buffers of nbytes: the practical ceilings of the write-bound
+    spindown pass 1 and the read-bound pass 2."""
+    g = (C.c_double * 6)()
+    L.check(L.load().tdf_memory_ceilings(nbytes, reps, g), "tdf_memory_ceilings")
+    return {"read": max(g[0], g[1]), "write": max(g[2], g[3]), "copy": max(g[4], g[5]),
+            "all": {"read": g[0], "read_nt": g[1], "write": g[2], "write_nt": g[3], "copy": g[4], "copy_nt": g[5]},
+            "buffer_bytes": nbytes}
+
+
 def _records(recs, n, pairs):
     out = []
     for k in range(n):

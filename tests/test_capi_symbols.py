@@ -40,3 +40,11 @@ def test_search_library_exports_every_symbol():
     lib = L.load_search()
     for name in names:
         assert hasattr(lib, name), name
+
+
+def test_memory_ceilings_rejects_bad_sizes_without_gpu():
+    lib = L.load()
+    g = (ctypes.c_double * 6)()
+    for nbytes, reps in ((0, 1), (17, 1), (1 << 20, 0)):
+        assert lib.tdf_memory_ceilings(nbytes, reps, g) == L.TDF_ERR_ARG
+    assert lib.tdf_memory_ceilings(1 << 20, 1, None) == L.TDF_ERR_ARG
