@@ -112,8 +112,10 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
         ctx.set_plan(*plan)
         print(f"[bench] prec={prec} plan {ctx.plan_info()}", file=sys.stderr, flush=True)
     elif tune:
+        t0 = time.perf_counter()
         p = ctx.autotune()
-        print(f"[bench] prec={prec} autotuned plan {p}", file=sys.stderr, flush=True)
+        print(f"[bench] prec={prec} autotuned plan {p} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+              flush=True)
     oc = opts.to_c()
     cap = (S + 1) * 64 * per_step
     npts = len(pts)
