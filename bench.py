@@ -11,6 +11,7 @@ measured in the same run.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2day-2det|6day-3det|2day-2det-small]
 """
 import argparse
+import copy
 import json
 import math
 import os
@@ -160,33 +161,77 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
     return ctx, opts, nt, el, kt, ntrig, ntp
 
 
-def fp32_vs_fp64_error(cfg, seg, pt, opts, ctx32, ctx64, nss=8):
-    """max / mean relative F error of the FP32 path vs the FP64 path over
-    bins with F64 >= 2 on nss spindowns, and trigger-set equality over the
-    full spindown range of sky point pt."""
+def _match_triggers(r32, r64, thr, tol):
+    """Relative F errors of the triggers both record lists hold (a trigger is
+    (record, bin), matched by frequency within tol) and whether the sets
+    agree, leaving out triggers within 1e-3 relative of thr."""
+    same = len(r32) == len(r64)
+    errs = []
+    for x, y in zip(r32, r64):
+        if (x.m, x.n, x.s) != (y.m, y.n, y.s):
+            same = False
+            continue
+        fx = np.array(sorted((float(f), float(v)) for f, v in x.ffstat)).reshape(-1, 2)
+        fy = np.array(sorted((float(f), float(v)) for f, v in y.ffstat)).reshape(-1, 2)
+        used = np.zeros(len(fy), bool)
+        for f, v in fx:
+            j = int(np.searchsorted(fy[:, 0], f)) if len(fy) else 0
+            cand = [k for k in (j - 1, j) if 0 <= k < len(fy) and not used[k] and abs(fy[k, 0] - f) <= tol]
+            if cand:
+                used[cand[0]] = True
+                errs.append(abs(v - fy[cand[0], 1]) / fy[cand[0], 1])
+            elif abs(v - thr) > 1e-3 * thr:
+                same = False  # an FP32 trigger the FP64 path does not have
+        for k in np.nonzero(~used)[0]:
+            if abs(fy[k, 1] - thr) > 1e-3 * thr:
+                same = False  # an FP64 trigger the FP32 path does not have
+    return errs, same
+
+
+def fp32_vs_fp64_error(cfg, seg, pts, opts, ctx32, ctx64, nsky=8, nss=8, thr_dense=12.0):
+    """FP32 path vs FP64 path: (i) the max relative F error over the matched
+    trigger set (full spindown range of the first nsky sky points at the
+    workload's thr, plus the first point at thr_dense for a larger set); (ii)
+    max / mean relative error over all in-band bins with F64 >= 2 of nss
+    spindowns; (iii) max |dF|/2 over those bins; and whether every trigger
+    set agrees, leaving out triggers within 1e-3 relative of thr."""
     S = cfg["nspin"]
     ctx32.keep_fstat(True)
     ctx64.keep_fstat(True)
-    sky32, r32 = ctx32.job_core(*pt, opts, max_spindowns=S + 1)
-    sky64, r64 = ctx64.job_core(*pt, opts, max_spindowns=S + 1)
-    thr = opts.thr
-    same = True
-    for x, y in zip(r32, r64):
-        fx = {round(float(f), 6) for f, v in x.ffstat if abs(v - thr) > 1e-3 * thr}
-        fy = {round(float(f), 6) for f, v in y.ffstat if abs(v - thr) > 1e-3 * thr}
-        same &= fx == fy
+    tol = 0.25 * 2 * math.pi / seg.sett.nfftf
+    terr, same, ntrig = [], True, {}
+    dense = copy.copy(opts)
+    dense.thr = thr_dense
+    for o, sel in ((opts, pts[:nsky]), (dense, pts[:1])):
+        n = 0
+        for pt in sel:
+            _, r32 = ctx32.job_core(*pt, o, max_spindowns=S + 1, pair_cap=(S + 1) * 512)
+            _, r64 = ctx64.job_core(*pt, o, max_spindowns=S + 1, pair_cap=(S + 1) * 512)
+            e, ok = _match_triggers(r32, r64, o.thr, tol)
+            terr += e
+            n += len(e)
+            same &= ok
+        ntrig[f"thr={o.thr:g}, {len(sel)} sky point(s)"] = n
     # F vectors of the same nss spindowns (one launch each, so both are the
     # last batch of their context)
+    pt = pts[0]
+    ctx32.job_core(*pt, opts, max_spindowns=S + 1)
+    ctx64.job_core(*pt, opts, max_spindowns=S + 1)
     i0 = S // 2 - nss // 2
     ctx32.spindown(i0, nss, opts)
     ctx64.spindown(i0, nss, opts)
-    errs = []
+    errs, dabs = [], 0.0
+    fr0, fr1 = opts.fr0, opts.fr1
     for iss in range(i0, i0 + nss):
         a, b = ctx32.fstat(iss), ctx64.fstat(iss)
+        a, b = a[fr0:fr1 + 1], b[fr0:fr1 + 1]
+        dabs = max(dabs, float(np.max(np.abs(a - b))) / 2)
         sel = b >= 2.0
         errs.append(np.abs(a[sel] - b[sel]) / b[sel])
     e = np.concatenate(errs) if errs else np.zeros(1)
-    return float(e.max()), float(e.mean()), bool(same)
+    return {"max_rel_err_F_triggers": float(max(terr)) if terr else None, "triggers_matched": ntrig,
+            "max_rel_err_F": float(e.max()), "mean_rel_err_F": float(e.mean()),
+            "max_abs_dF_half": dabs, "trigger_sets_equal": bool(same)}
 
 
 def cpu_baseline(cfg, seg, pts, budget_s=12.0):
@@ -213,7 +258,13 @@ def cpu_baseline(cfg, seg, pts, budget_s=12.0):
         el = time.perf_counter() - t0
         if el > budget_s or nsky >= 64:
             break
-    return dict(value=nt / el, unit="templates/s", cores=threads, kind="port",
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
+    return dict(value=nt / el, unit="templates/s", cores=threads, kind="port", cpu_model=model,
                 sample=f"{nsky} sky point(s) x {S} spindowns ({nt} templates) of the same workload, "
                        f"oracle job_core with PREC=FLOAT semantics, OpenMP over spindowns, "
                        f"{threads} threads, {el:.1f} s")
@@ -316,10 +367,9 @@ def main():
         value64 = nt64 / el64
         print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
               flush=True)
-        emax, emean, same = fp32_vs_fp64_error(cfg, seg, pts[0], opts, ctx32, ctx64)
+        fperr = fp32_vs_fp64_error(cfg, seg, pts, opts, ctx32, ctx64)
         fp64 = {"value": round(value64, 1), "speedup_fp32_over_fp64": round(value / value64, 3),
                 "plan": ctx64.plan_info()}
-        fperr = {"max_rel_err_F": emax, "mean_rel_err_F": emean, "trigger_sets_equal": same}
 
     # roofline of the dominant spindown kernel (HIP events on the library stream)
     spin = {k: v for k, v in kt.items() if k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat", "triggers")}
