@@ -253,7 +253,10 @@ int tdf_autotune(tdf_ctx *ctx);
  * and read with non-temporal accesses (set when a launch's Z goes to HBM,
  * clear when it fits the memory-side cache), bit 2 (FP32, with bit 0): the
  * fused pass 2 as a streaming grid that stages the next column in LDS while
- * it transforms the current one, batch = spindowns per launch */
+ * it transforms the current one, bit 3 (FP32, with bit 0, not with bit 2):
+ * the streaming grid staging half of the next column in LDS and loading the
+ * other half into registers (less LDS per workgroup, more workgroups per
+ * compute unit), batch = spindowns per launch */
 int tdf_plan_info(tdf_ctx *ctx, int *n1, int *e1, int *e2, int *fused, int *batch);
 /* selects a plan explicitly (same meaning; batch <= the creation batch) */
 int tdf_set_plan(tdf_ctx *ctx, int n1, int e1, int e2, int fused, int batch);

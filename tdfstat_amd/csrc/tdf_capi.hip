@@ -1387,13 +1387,13 @@ int autotune_search(tdf_ctx *c, hipEvent_t e0, hipEvent_t e1, FftPlan &best, int
     for (int n1 : n1s)
         for (int ea : es)
             for (int eb : es)
-                for (int f2 = 0; f2 < 3; f2++)
+                for (int f2 = 0; f2 < 4; f2++)
                 for (int ntz = 0; ntz < 2; ntz++) {
                 if (ea > emax || eb > emax) continue;
                 FftPlan p{};
                 if (make_plan(c, n, &p, n1, ea, eb) != TDF_OK) continue;
                 if (f2 && (eb != 2 * n1 || n1 < 512 || (d && n1 > 1024))) continue;
-                if (f2 == 2 && d) continue;  // the streaming pass 2 is FP32
+                if (f2 >= 2 && d) continue;  // the streaming pass 2 is FP32
                 p.f2 = f2;
                 p.ntz = ntz;
                 for (int b : bs) {
@@ -1476,7 +1476,8 @@ int tdf_plan_info(tdf_ctx *c, int *n1, int *e1, int *e2, int *fused, int *batch)
     if (n1) *n1 = c->p_spin.n1;
     if (e1) *e1 = c->p_spin.e1;
     if (e2) *e2 = c->p_spin.e2;
-    if (fused) *fused = (c->p_spin.f2 ? 1 : 0) | (c->p_spin.ntz << 1) | (c->p_spin.f2 == 2 ? 4 : 0);
+    if (fused)
+        *fused = (c->p_spin.f2 ? 1 : 0) | (c->p_spin.ntz << 1) | (c->p_spin.f2 == 2 ? 4 : 0) | (c->p_spin.f2 == 3 ? 8 : 0);
     if (batch) *batch = c->batch;
     return TDF_OK;
 }
@@ -1489,8 +1490,8 @@ int tdf_set_plan(tdf_ctx *c, int n1, int e1, int e2, int fused, int batch)
     int rc = make_plan(c, c->g.nfftf, &p, n1, e1, e2);
     if (rc) return rc;
     if ((fused & 1) && (p.e2 != 2 * p.n1 || p.n1 < 512 || (d && p.n1 > 1024))) return TDF_ERR_UNSUPPORTED;
-    if ((fused & 4) && (!(fused & 1) || d)) return TDF_ERR_UNSUPPORTED;
-    p.f2 = (fused & 1) ? ((fused & 4) ? 2 : 1) : 0;
+    if ((fused & 12) && (!(fused & 1) || d || (fused & 12) == 12)) return TDF_ERR_UNSUPPORTED;
+    p.f2 = (fused & 1) ? ((fused & 8) ? 3 : (fused & 4) ? 2 : 1) : 0;
     p.ntz = (fused >> 1) & 1;
     HIPCHK(hipStreamSynchronize(c->stream));
     if (p.n1 != c->p_spin.n1) {

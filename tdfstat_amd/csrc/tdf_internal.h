@@ -109,7 +109,8 @@ __device__ inline void modvir_sample(const double *k, const ModvirSky &m, double
 struct FftPlan {
     int n, n1, n2, l1, l2;
     int e1, e2;  // LDS tile elements of pass 1 / pass 2
-    int f2;      // spindown pass 2: fused first/last stages (needs e2 == 2*n1); 2: streaming grid
+    int f2;      // spindown pass 2: fused first/last stages (needs e2 == 2*n1); 2: streaming grid;
+                 // 3: streaming grid staging half of each column
     int ntz;     // spindown Z accesses non-temporal (1) or ordinary (0)
     const void *twL[2][16];  // [is_double][log2 L] -> W_L^m, m < L
     const void *tw_lo[2], *tw_hi[2];

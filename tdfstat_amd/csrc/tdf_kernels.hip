@@ -488,7 +488,8 @@ struct IO_SpinP2 {
 // The column's 2*L values are read through a buffer resource based at the
 // column (one offset register per lane, the row step r*L/16 as a scalar
 // offset), so the loads need no per-load 64-bit addresses.
-template <typename T, int L, bool NTZ>
+// R0, R1: only v[r - R0] for R0 <= r < R1 (rows j + r*L/16) are loaded.
+template <typename T, int L, bool NTZ, int R0 = 0, int R1 = 16>
 __device__ __forceinline__ void pass2_load(const IO_SpinP2<T, NTZ> &io, int slot, int k2, cx<T> *v)
 {
     const cx<T> *zc = io.Z + (size_t)slot * 2 * io.n + (size_t)k2 * io.n1 * 2;
@@ -498,10 +499,11 @@ __device__ __forceinline__ void pass2_load(const IO_SpinP2<T, NTZ> &io, int slot
     const int vo = (2 * j + c) * (int)sizeof(cx<T>);
     constexpr int AUX = NTZ ? BUF_NT : 0;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
+    for (int r = R0; r < R1; r++) {
         const int so = r * (L / 16) * 2 * (int)sizeof(cx<T>);
-        if constexpr (sizeof(T) == 4) v[r] = __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, AUX));
-        else v[r] = __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(rs, vo, so, AUX));
+        if constexpr (sizeof(T) == 4)
+            v[r - R0] = __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, AUX));
+        else v[r - R0] = __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(rs, vo, so, AUX));
     }
 }
 
@@ -509,14 +511,16 @@ __device__ __forceinline__ void pass2_load(const IO_SpinP2<T, NTZ> &io, int slot
 // can hold bins of the band and the last stage forms only those outputs.
 // raa, rbb: 1/aa and 1/bb of the launch's sky point (F = |Fa|^2/aa + |Fb|^2/bb).
 // after_first() runs once the first stage's barrier has passed: every
-// wavefront has then consumed its inputs v and written the tile.
+// wavefront has then consumed its inputs v and written the tile;
+// after_last() runs when the last stage has formed its outputs in
+// registers, before the F epilogue.
 struct no_op {
     __device__ void operator()() const {}
 };
-template <typename T, int L, bool NTZ, bool LO, class AF = no_op>
+template <typename T, int L, bool NTZ, bool LO, class AF = no_op, class AL = no_op>
 __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const cx<T> *__restrict__ twL,
                                              cx<T> *tile, int slot, int k2, cx<T> *v, T raa, T rbb,
-                                             AF after_first = AF())
+                                             AF after_first = AF(), AL after_last = AL())
 {
     constexpr int NCOL = 2, NT = L / 8, RS = L / 256, NB = 16 / RS, LR = L / RS;
     constexpr int RO = LO ? RS / 2 : RS;  // outputs per butterfly that are used
@@ -524,6 +528,7 @@ __device__ __forceinline__ void pass2_column(const IO_SpinP2<T, NTZ> &io, const 
     after_first();
     stockham_stage<T, -1, L, NCOL, 16, 16>(tile, twL);
     stockham_last_regs<T, -1, L, NCOL, RS, 256, LO>(tile, twL, v);
+    after_last();
     const T thrT = (T)io.thr;
     T *ft = io.FT + (size_t)slot * io.n + (size_t)k2 * io.n1;
     // flag words of the column (its first bin is a multiple of 64: n1 >= 512)
@@ -585,17 +590,21 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io
 // first stage's barrier (by then every wavefront has read the current
 // column from the staging area), so it runs while the column is
 // transformed.
-template <int L>
+// HALF: only rows q < L/2 of the column are staged (the other half is
+// loaded straight into registers).
+template <int L, bool HALF = false>
 struct stream_geom {
     static constexpr int NT = L / 8, W = NT / 64;
-    static constexpr int CHUNKS = 2 * L * 8 / 1024;  // 1 KiB chunks of a column
-    static constexpr int PER_WAVE = CHUNKS / W;      // = 8
+    static constexpr int ROWS = HALF ? L / 2 : L;    // staged rows (16 bytes each)
+    static constexpr int CHUNKS = ROWS * 16 / 1024;  // 1 KiB chunks
+    static constexpr int PER_WAVE = CHUNKS / W;      // 8 (HALF: 4)
+    static_assert(CHUNKS % W == 0, "whole chunks per wavefront");
 };
 
-template <typename T, int L, bool NTZ>
+template <typename T, int L, bool NTZ, bool HALF = false>
 __device__ __forceinline__ void stream_issue(const IO_SpinP2<T, NTZ> &io, int item, cx<T> *raw)
 {
-    using G = stream_geom<L>;
+    using G = stream_geom<L, HALF>;
     const int slot = item / io.n2, k2 = item % io.n2;
     const cx<T> *zc = io.Z + (size_t)slot * 2 * io.n + (size_t)k2 * io.n1 * 2;
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(zc, (size_t)2 * L * sizeof(cx<T>));
@@ -634,6 +643,48 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_stream(IO_SpinP2<T, NTZ> i
             // the staging area is read: the next copy may land
             if (nxt < nitems) stream_issue<T, L, NTZ>(io, nxt, raw);
         });
+    }
+}
+
+// Streaming form with half the column staged: rows q < L/2 of the next
+// column are copied into LDS while the current column is transformed (as
+// above); rows q >= L/2 (v[8..15]) are loaded into registers once the last
+// stage has formed its outputs, so their latency overlaps the F epilogue.
+// The LDS per workgroup drops from tile + L rows to tile + L/2 rows, so
+// three workgroups of 2048-row columns fit a compute unit instead of two
+// (three wavefronts per SIMD, within 168 VGPRs).
+template <typename T, int L, bool NTZ, bool LO>
+__global__ void __launch_bounds__(L / 8) __attribute__((amdgpu_waves_per_eu(3)))
+k_spin_pass2_stream_half(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL, int nitems)
+{
+    static_assert(sizeof(T) == 4, "FP32 columns of 16 bytes per row");
+    __shared__ cx<T> tile[2 * ldsgeom<L>::LP];
+    __shared__ cx<T> raw[L];  // rows q < L/2, a and b interleaved
+    int item = blockIdx.x;
+    cx<T> nv[8];
+    if (item < nitems) {
+        stream_issue<T, L, NTZ, true>(io, item, raw);
+        pass2_load<T, L, NTZ, 8, 16>(io, item / io.n2, item % io.n2, nv);
+    }
+    const int c = threadIdx.x & 1, j = threadIdx.x >> 1;
+    const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
+    for (; item < nitems; item += gridDim.x) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        cx<T> v[16];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = raw[2 * (j + r * (L / 16)) + c];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[8 + r] = nv[r];
+        const int nxt = item + gridDim.x;
+        pass2_column<T, L, NTZ, LO>(
+            io, twL, tile, item / io.n2, item % io.n2, v, raa, rbb,
+            [&]() {
+                if (nxt < nitems) stream_issue<T, L, NTZ, true>(io, nxt, raw);
+            },
+            [&]() {
+                if (nxt < nitems) pass2_load<T, L, NTZ, 8, 16>(io, nxt / io.n2, nxt % io.n2, nv);
+            });
     }
 }
 
@@ -1196,31 +1247,37 @@ hipError_t launch_spin_pass1(const FftPlan &p, const SpinArgs &s, hipStream_t st
 
 // Streaming pass 2: as many workgroups as are resident at once on the
 // device (occupancy x compute units), found once per kernel variant.
-template <typename T, int L, bool NTZ, bool LO>
+template <typename T, int L, bool NTZ, bool LO, bool HALF>
 static hipError_t launch_stream_L(const IO_SpinP2<T, NTZ> &io, const cx<T> *tw, int nitems, hipStream_t st)
 {
     static int resident = 0;
     if (resident <= 0) {
         int nb = 0, dev = 0, ncu = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spin_pass2_stream<T, L, NTZ, LO>, L / 8, 0);
+        hipError_t e = HALF ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spin_pass2_stream_half<T, L, NTZ, LO>,
+                                                                           L / 8, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spin_pass2_stream<T, L, NTZ, LO>,
+                                                                           L / 8, 0);
         if (e == hipSuccess) e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
         resident = (nb > 0 ? nb : 1) * (ncu > 0 ? ncu : 1);
     }
     const int grid = resident < nitems ? resident : nitems;
-    hipLaunchKernelGGL((k_spin_pass2_stream<T, L, NTZ, LO>), dim3(grid), dim3(L / 8), 0, st, io, tw, nitems);
+    if constexpr (HALF)
+        hipLaunchKernelGGL((k_spin_pass2_stream_half<T, L, NTZ, LO>), dim3(grid), dim3(L / 8), 0, st, io, tw, nitems);
+    else
+        hipLaunchKernelGGL((k_spin_pass2_stream<T, L, NTZ, LO>), dim3(grid), dim3(L / 8), 0, st, io, tw, nitems);
     return hipGetLastError();
 }
 
-template <typename T, bool NTZ, bool LO>
+template <typename T, bool NTZ, bool LO, bool HALF>
 static hipError_t launch_stream(int n1, const IO_SpinP2<T, NTZ> &io, const cx<T> *tw, int nitems, hipStream_t st)
 {
     if constexpr (sizeof(T) == 4) {
         switch (n1) {
-        case 512: return launch_stream_L<T, 512, NTZ, LO>(io, tw, nitems, st);
-        case 1024: return launch_stream_L<T, 1024, NTZ, LO>(io, tw, nitems, st);
-        case 2048: return launch_stream_L<T, 2048, NTZ, LO>(io, tw, nitems, st);
+        case 512: return launch_stream_L<T, 512, NTZ, LO, HALF>(io, tw, nitems, st);
+        case 1024: return launch_stream_L<T, 1024, NTZ, LO, HALF>(io, tw, nitems, st);
+        case 2048: return launch_stream_L<T, 2048, NTZ, LO, HALF>(io, tw, nitems, st);
         default: break;
         }
     }
@@ -1259,10 +1316,13 @@ static hipError_t launch_spin_pass2_z(const FftPlan &p, const SpinArgs &s, hipSt
         const dim3 grid(p.n2, s.nslots);
         // bins k = k2 + n2*k1 < n/2 have k1 < n1/2
         const bool lo = s.fr1 < p.n / 2;
-        if (p.f2 == 2) {
+        if (p.f2 == 2 || p.f2 == 3) {
             const int nitems = p.n2 * s.nslots;
-            return lo ? launch_stream<T, NTZ, true>(p.n1, io, tw, nitems, st)
-                      : launch_stream<T, NTZ, false>(p.n1, io, tw, nitems, st);
+            if (p.f2 == 3)
+                return lo ? launch_stream<T, NTZ, true, true>(p.n1, io, tw, nitems, st)
+                          : launch_stream<T, NTZ, false, true>(p.n1, io, tw, nitems, st);
+            return lo ? launch_stream<T, NTZ, true, false>(p.n1, io, tw, nitems, st)
+                      : launch_stream<T, NTZ, false, false>(p.n1, io, tw, nitems, st);
         }
         if (lo) return launch_fused<T, NTZ, true>(p.n1, grid, io, tw, st);
         return launch_fused<T, NTZ, false>(p.n1, grid, io, tw, st);

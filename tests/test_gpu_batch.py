@@ -69,15 +69,20 @@ def test_sparse_f_gives_the_same_records(prec, thr):
     sett = seg.sett
     opts = TS.default_opts(sett, thr=thr, sst=-6, spndr1=6, sstep=1)
     pts = POINTS[:2] + POINTS[3:]
-    out = []
-    for keep in (True, False):
-        with TS.Context(sett, seg.dets, prec=prec, batch=8, keep_fstat=keep) as ctx:
-            plan = (512, 2048 if prec == L.TDF_PREC_FP64 else 4096, 1024, 1)
-            ctx.set_plan(*plan, 8)
-            out.append(ctx.job_core_batch(pts, opts, pair_cap=1 << 21))
-    (_, n0, r0), (_, n1, r1) = out
-    assert list(n0) == list(n1) and len(r0) == len(r1)
-    for a, b in zip(r0, r1):
-        assert (a.m, a.n, a.s) == (b.m, b.n, b.s)
-        np.testing.assert_array_equal(a.ffstat, b.ffstat)
-    _flush(f"  {sum(len(r.ffstat) for r in r0)} triggers")
+    # the fused pass 2, and (FP32) both streaming forms
+    sparse_plans = [(512, 2048 if prec == L.TDF_PREC_FP64 else 4096, 1024, 1)]
+    if prec == L.TDF_PREC_FP32:
+        sparse_plans += [(2048, 2048, 4096, 1, 1, 1), (2048, 2048, 4096, 1, 1, 2)]
+    for plan in sparse_plans:
+        _flush(f"  plan {plan}")
+        out = []
+        for keep in (True, False):
+            with TS.Context(sett, seg.dets, prec=prec, batch=8, keep_fstat=keep) as ctx:
+                ctx.set_plan(*plan[:4], 8, *plan[4:])
+                out.append(ctx.job_core_batch(pts, opts, pair_cap=1 << 21))
+        (_, n0, r0), (_, n1, r1) = out
+        assert list(n0) == list(n1) and len(r0) == len(r1)
+        for a, b in zip(r0, r1):
+            assert (a.m, a.n, a.s) == (b.m, b.n, b.s)
+            np.testing.assert_array_equal(a.ffstat, b.ffstat)
+        _flush(f"  {sum(len(r.ffstat) for r in r0)} triggers")
