@@ -392,6 +392,16 @@ def main():
         a_ = kernel_bytes(sett, S, nslots_avg, k, False) / (kt[k][0] / kt[k][1] * 1e-3) / 1e9
         passes[k] = {"achieved": round(a_, 1), "frac": round(a_ / HBM_PEAK_GBS, 4), "ceiling_kind": kind,
                      "frac_of_ceiling": round(a_ / ceil[kind], 4) if ceil else None}
+    # pass 1 writes the four-step intermediate Z and pass 2 reads it back:
+    # together a copy of Z through HBM, bounded by the copy ceiling
+    pair = None
+    if all(k in kt for k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat")):
+        b_pair = sum(kernel_bytes(sett, S, nslots_avg, k, False)
+                     for k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat"))
+        t_pair = sum(kt[k][0] / kt[k][1] for k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat")) * 1e-3
+        a_ = b_pair / t_pair / 1e9
+        pair = {"bytes_per_launch_pair": int(b_pair), "achieved": round(a_, 1), "frac": round(a_ / HBM_PEAK_GBS, 4),
+                "ceiling_kind": "copy", "frac_of_ceiling": round(a_ / ceil["copy"], 4) if ceil else None}
     line = {
         "metric": "templates/s (sky x spindown), FP32 bulk + FP64 phase",
         "value": round(value, 1),
@@ -417,6 +427,7 @@ def main():
         "pipeline_roofline": {"bytes_per_template": int(bt), "achieved": round(pipe, 1),
                               "frac": round(pipe / HBM_PEAK_GBS, 4)},
         "passes": passes,
+        "spindown_pair": pair,
         "memory_ceilings": ceil,
         "fp64_path": fp64,
         "fp32_vs_fp64": fperr,
