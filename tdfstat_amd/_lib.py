@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libtdfstat_hip.so")
 
 TDF_OK = 0
 TDF_ERR_ARG = -1
+TDF_ERR_UNSUPPORTED = -5
 TDF_PREC_FP32 = 0
 TDF_PREC_FP64 = 1
 

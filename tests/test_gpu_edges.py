@@ -1,0 +1,92 @@
+"""GPU edge cases of job_core beyond the parity suite: veto lines inside the
+band (a band's line list, applied with veto_flag; common.c:397-402, the
+trigger loop jobcore.c:558-582) and a one-detector network."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tdfstat_amd import _lib as L
+from tdfstat_amd import search as TS
+from tdfstat_amd import synth
+
+from test_gpu_parity import PRECS, _compare_templates, _flush, _segment
+
+pytestmark = pytest.mark.gpu
+
+
+def _pairs(recs):
+    return {(r.s, round(float(f), 6)) for r in recs for f, _ in r.ffstat}
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_interior_veto_lines(prec, tol):
+    """Lines inside the band remove exactly the triggers whose frequency
+    falls in one of them: a vetoed local maximum still skips its right
+    neighbour (jobcore.c:565-582), so the set with the lines is the set
+    without them minus the vetoed ones.  The F arrays and trigger sets also
+    equal the oracle's with the same lines."""
+    _flush(f"interior veto lines prec={prec}")
+    inj = synth.Injection(snr=40.0, f=1.3, fdot=0.0, ra=1.0, dec=0.3, iota=0.5, psi=0.2, phase=0.1)
+    seg = _segment(injections=[inj])
+    sett = seg.sett
+    thr = 8.0
+    base = TS.default_opts(sett, thr=thr, sst=-1, spndr1=1, sstep=1)
+    with TS.Context(sett, seg.dets, prec=prec, batch=8) as ctx:
+        _, r0 = ctx.job_core(1, 1.0, 1.0, base)
+        fs = sorted({float(f) for r in r0 for f, _ in r.ffstat})
+        assert len(fs) >= 10, len(fs)
+        # a narrow line on every third trigger frequency, and one wide line
+        # over an eighth of the band
+        w = 0.25 * 2 * np.pi / sett.nfftf
+        extra = [(f - w, f + w) for f in fs[::3]] + [(1.0, 1.0 + np.pi / 8)]
+        opts = TS.default_opts(sett, thr=thr, sst=-1, spndr1=1, sstep=1)
+        opts.lines = tuple(base.lines) + tuple(extra)
+        _, r1 = ctx.job_core(1, 1.0, 1.0, opts)
+        vetoed = {(s, f) for s, f in _pairs(r0) if any(a <= f <= b for a, b in extra)}
+        assert vetoed, "no trigger falls in the lines"
+        assert _pairs(r1) == _pairs(r0) - vetoed
+        _flush(f"  {len(_pairs(r0))} triggers, {len(vetoed)} vetoed")
+        orc = O.Oracle(sett, seg.dets)
+        orc.sky_stage(1, 1.0, 1.0, opts)
+        _compare_templates(ctx, orc, r1, opts, 3, tol, thr)
+
+
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_one_detector(prec, tol):
+    """nifo = 1: the sky stage, normalisation sums and spindown loop with a
+    single detector equal the oracle's."""
+    _flush(f"one detector prec={prec}")
+    inj = synth.Injection(snr=30.0, f=2.1, fdot=0.0, ra=0.4, dec=-0.6, iota=0.3, psi=0.7, phase=1.1)
+    seg = _segment(dets=("H1",), injections=[inj], gap_fracs=(0.1,))
+    sett = seg.sett
+    assert sett.nifo == 1
+    opts = TS.default_opts(sett, thr=10.0, sst=-1, spndr1=1, sstep=1)
+    orc = O.Oracle(sett, seg.dets)
+    so = orc.sky_stage(1, 1.0, 1.0, opts)["sky"]
+    assert so.visible == 1
+    with TS.Context(sett, seg.dets, prec=prec, batch=8) as ctx:
+        sky, recs = ctx.job_core(1, 1.0, 1.0, opts)
+        assert sky.visible == 1 and len(recs) == 3
+        assert abs(sky.aa - so.aa) <= 1e-9 * abs(so.aa)
+        assert abs(sky.bb - so.bb) <= 1e-9 * abs(so.bb)
+        worst = _compare_templates(ctx, orc, recs, opts, 3, tol, 10.0)
+        _flush(f"  max rel F err={worst:.3e}, {sum(len(r.ffstat) for r in recs)} triggers")
+
+
+def test_set_plan_rejects_unsupported_forms():
+    """The streaming pass-2 forms are FP32 only, exclusive of each other and
+    need the fused pass 2 (tdf_set_plan bits 2 and 3)."""
+    _flush("set_plan forms")
+    seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=3)  # nfftf 2^17
+    lib = L.load()
+    with TS.Context(seg.sett, seg.dets, prec=L.TDF_PREC_FP64, batch=8) as c64:
+        c64.set_plan(1024, 1024, 2048, 1, 8, 1, 0)
+        for stream in (1, 2):
+            with pytest.raises(L.TdfError):
+                c64.set_plan(1024, 1024, 2048, 1, 8, 1, stream)
+    with TS.Context(seg.sett, seg.dets, prec=L.TDF_PREC_FP32, batch=8) as c32:
+        assert lib.tdf_set_plan(c32.h, 2048, 2048, 4096, 1 | 2 | 4 | 8, 8) == L.TDF_ERR_UNSUPPORTED
+        assert lib.tdf_set_plan(c32.h, 2048, 2048, 4096, 8, 8) == L.TDF_ERR_UNSUPPORTED  # no fused bit
+        assert lib.tdf_set_plan(c32.h, 2048, 2048, 4096, 1, 9) == L.TDF_ERR_ARG  # above the creation batch
+        c32.set_plan(2048, 2048, 4096, 1, 8, 1, 2)
+        assert c32.plan_info()["stream"] == 2
