@@ -1,0 +1,124 @@
+"""A data segment in the reference's input directory layout: reading it the
+way the search's set-up does, and writing it (the role of the reference's
+segment generators).
+
+Layout (per segment SSS = seg, band BBBB, detector D in {H1, L1, V1}):
+
+  <indir>/SSS/D/xdat_SSS_BBBB.bin      N float32 (read_O3: xdatsc_..., N float64)
+  <indir>/SSS/D/DetSSB.bin             3N float64 positions, then phir, epsm
+  <indir>/SSS/D/starting_date          start time (GPS s), text
+  <indir>/SSS/grids/grid_SSS_BBBB_<D...>.bin   int32 fftpad, then float64 M[16]
+                                       (one detector: <indir>/SSS/D/grid.bin)
+
+Reading follows main.c's order: detectors_settings (settings.c:115-176:
+the H1, L1, V1 subdirectories that exist, restricted by usedet, each of which
+must hold its data file), read_grid (init.c:148-197: fftpad and M),
+search_settings (settings.c:20-105, with that fftpad), then per detector
+read_xdat (common.c:42-80: zero count, sig2 = N/(N-Nzeros)*var), read_detssb
+(common.c:88-128), read_start_time (common.c:133-152) and rogcvir
+(settings.c:226-248).  A missing directory or file raises, where the
+reference prints and exits; so does a data or ephemeris file shorter than N
+samples, which the reference would read as zeros past its end.
+"""
+import os
+
+import numpy as np
+
+from . import settings as S
+from . import synth
+
+
+def _segdir(indir, seg):
+    return os.path.join(indir, f"{seg:03d}")
+
+
+def xdat_path(indir, seg, band, det, read_o3=False):
+    name = f"xdatsc_{seg:03d}_{band:04d}.bin" if read_o3 else f"xdat_{seg:03d}_{band:04d}.bin"
+    return os.path.join(_segdir(indir, seg), det, name)
+
+
+def grid_path(indir, seg, band, dets, grid_file=""):
+    """init.c:162-178: an absolute grid_file, or one relative to indir, or
+    the one-detector / network default."""
+    if grid_file:
+        return grid_file if os.path.isabs(grid_file) else os.path.join(indir, grid_file)
+    if len(dets) == 1:
+        return os.path.join(_segdir(indir, seg), dets[0], "grid.bin")
+    return os.path.join(_segdir(indir, seg), "grids", f"grid_{seg:03d}_{band:04d}_{''.join(dets)}.bin")
+
+
+def find_detectors(indir, seg, band, usedet="", read_o3=False):
+    """settings.c:115-176: detectors in the order H1, L1, V1 whose directory
+    exists (only those named in usedet when it is given); a detector
+    directory without its data file, or a usedet detector without a
+    directory, is an error."""
+    sd = _segdir(indir, seg)
+    if not os.path.isdir(sd):
+        raise FileNotFoundError(f"can't open the input directory {sd}")
+    found = []
+    for det in S.DET_ORDER:
+        if usedet and det not in usedet:
+            continue
+        dd = os.path.join(sd, det)
+        if os.path.isdir(dd):
+            x = xdat_path(indir, seg, band, det, read_o3)
+            if not os.path.isfile(x):
+                raise FileNotFoundError(f"directory {dd} exists, but no input file {x}")
+            found.append(det)
+        elif usedet:
+            raise FileNotFoundError(f"can't open the input directory required by usedet: {dd}")
+    if not found:
+        raise FileNotFoundError(f"no detector data under {sd}")
+    return found
+
+
+def load_segment(indir, seg, band, nod, dt=2.0, overlap=0.03125, usedet="", grid_file="", read_o3=False,
+                 N=None):
+    """One segment as the search reads it -> synth.Segment (no injections).
+    N overrides the sidereal-day length, as synth.make_segment's N does
+    (small tests)."""
+    dets = find_detectors(indir, seg, band, usedet, read_o3)
+    fftpad, M = S.read_grid(grid_path(indir, seg, band, dets, grid_file))
+    sett = S.SearchSettings(fpo=S.fpo(band, overlap, dt), dt=dt, nod=nod, fftpad=fftpad)
+    S.search_settings(sett)
+    if N is not None:
+        synth.override_length(sett, N)
+    sett.nifo = len(dets)
+    sett.M = np.array(M, dtype=np.float64)
+    out = []
+    for det in dets:
+        dd = os.path.join(_segdir(indir, seg), det)
+        x = S.read_xdat(xdat_path(indir, seg, band, det, read_o3), sett.N, read_o3)
+        if x.size != sett.N:
+            raise ValueError(f"{det}: {x.size} samples in the data file, N = {sett.N}")
+        ssb, phir, epsm = S.read_detssb(os.path.join(dd, "DetSSB.bin"), sett.N)
+        if ssb.size != 3 * sett.N:
+            raise ValueError(f"{det}: DetSSB.bin holds fewer than 3N + 2 doubles")
+        with open(os.path.join(dd, "starting_date")) as f:
+            start = float(f.read().split()[0])
+        lat, _lon, _h, gam = S.DETECTORS[det]
+        d = synth.Detector(name=det, xDat=x, DetSSB=ssb, phir=phir, epsm=epsm, sigma=0.0,
+                           amod=S.rogcvir(lat, gam), start_time=start)
+        d.sig2 = S.xdat_sig2(d.xDat)[2]
+        out.append(d)
+        # read_detssb sets the network's obliquity from each detector in turn
+        sett.sepsm, sett.cepsm = np.sin(epsm), np.cos(epsm)
+    return synth.Segment(sett=sett, dets=out, injections=[])
+
+
+def write_segment(segment, indir, seg, band, start_time=None):
+    """Write a segment in the layout above (xdat float32); returns the grid
+    file's path."""
+    names = [d.name for d in segment.dets]
+    for d in segment.dets:
+        dd = os.path.join(_segdir(indir, seg), d.name)
+        os.makedirs(dd, exist_ok=True)
+        np.asarray(d.xDat, dtype=np.float32).tofile(xdat_path(indir, seg, band, d.name))
+        S.write_detssb(os.path.join(dd, "DetSSB.bin"), d.DetSSB, d.phir, d.epsm)
+        t0 = d.start_time if start_time is None else start_time
+        with open(os.path.join(dd, "starting_date"), "w") as f:
+            f.write(f"{t0:.6f}\n")
+    gp = grid_path(indir, seg, band, names)
+    os.makedirs(os.path.dirname(gp), exist_ok=True)
+    S.write_grid(gp, segment.sett.fftpad, segment.sett.M)
+    return gp

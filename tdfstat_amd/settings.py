@@ -163,7 +163,7 @@ def default_narrowdown(overlap):
     return (1.0 - overlap) / 2.0 * math.pi
 
 
-# ---- file formats (Appendix: reference on-disk layout) ----
+# ---- the reference's input file formats ----
 
 def write_grid(path, fftpad, M):
     with open(path, "wb") as f:

@@ -168,6 +168,18 @@ class Segment:
     injections: list
 
 
+def override_length(sett, N):
+    """Settings of a segment of N samples instead of nod sidereal days
+    (small tests): the lengths and band edges search_settings derives
+    from N."""
+    sett.N = int(N)
+    sett.nfft = 1 << int(math.ceil(math.log2(sett.N)))
+    sett.Ninterp = 2 * sett.nfft
+    sett.nfftf = sett.fftpad * sett.nfft
+    sett.nmin = int(sett.fftpad * S.NAV * sett.B)
+    sett.nmax = int((sett.nfft // 2 - S.NAV * sett.B) * sett.fftpad)
+
+
 def make_segment(nod=2, dt=2.0, band=372, overlap=0.03125, det_names=("H1", "L1"), sigmas=None,
                  gap_fracs=None, fftpad=1, seed=1, injections=(), M=None, N=None):
     """Seeded segment.  N overrides the sidereal-day length (small tests).
@@ -177,12 +189,7 @@ def make_segment(nod=2, dt=2.0, band=372, overlap=0.03125, det_names=("H1", "L1"
     sett = S.SearchSettings(fpo=S.fpo(band, overlap, dt), dt=dt, nod=nod, fftpad=fftpad)
     S.search_settings(sett)
     if N is not None:
-        sett.N = int(N)
-        sett.nfft = 1 << int(math.ceil(math.log2(sett.N)))
-        sett.Ninterp = 2 * sett.nfft
-        sett.nfftf = fftpad * sett.nfft
-        sett.nmin = int(sett.fftpad * S.NAV * sett.B)
-        sett.nmax = int((sett.nfft // 2 - S.NAV * sett.B) * sett.fftpad)
+        override_length(sett, N)
     sett.nifo = len(det_names)
     sett.M = np.array(M if M is not None else scaled_grid(sett.N, sett.nfftf), dtype=np.float64)
     gmst0 = rng.uniform(0, 2 * math.pi)
