@@ -8,7 +8,19 @@ FP64-phase path; the plain FP64 path, the FP32-vs-FP64 F-statistic error and
 a CPU baseline (the oracle, reference PREC=FLOAT semantics, OpenMP) are
 measured in the same run.
 
+Besides the required fields the JSON line carries: roofline (the dominant
+spindown kernel, algorithmic bytes per launch over its HIP-event time,
+against 8 TB/s, with the rocprof-measured traffic of the same plan when the
+committed counter summary holds it), pipeline_roofline (all algorithmic
+bytes per template x templates/s), memory_ceilings (plain read / write /
+copy kernels on 2 GiB buffers), passes (each spindown pass against 8 TB/s
+and against the ceiling of its access kind), spindown_pair (pass 1 + pass 2,
+a copy of the four-step intermediate through HBM, against the copy
+ceiling), fp64_path, fp32_vs_fp64 (matched-trigger, all-bin and absolute F
+errors, trigger-set equality), kernels_ms and cpu_baseline.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2day-2det|6day-3det|2day-2det-small]
+  python bench.py --ceilings-only
 """
 import argparse
 import copy
