@@ -17,7 +17,9 @@ copy kernels on 2 GiB buffers), passes (each spindown pass against 8 TB/s
 and against the ceiling of its access kind), spindown_pair (pass 1 + pass 2,
 a copy of the four-step intermediate through HBM, against the copy
 ceiling), fp64_path, fp32_vs_fp64 (matched-trigger, all-bin and absolute F
-errors, trigger-set equality), kernels_ms and cpu_baseline.
+errors, trigger-set equality), setup (context creation with the segment
+upload, autotune and warm-up times, outside the timed steps), kernels_ms and
+cpu_baseline.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2day-2det|6day-3det|2day-2det-small]
   python bench.py --ceilings-only
@@ -120,14 +122,19 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
     S = cfg["nspin"]
     opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
     # a search keeps no F arrays: F only at bins >= thr (the trigger rule's input)
+    setup = {}
+    t0 = time.perf_counter()
     ctx = TS.Context(sett, seg.dets, prec=prec, batch=max(batch, plan[4] if plan else 0), keep_fstat=False)
+    ctx.synchronize()
+    setup["context_and_upload_s"] = round(time.perf_counter() - t0, 4)
     if plan:
         ctx.set_plan(*plan)
         print(f"[bench] prec={prec} plan {ctx.plan_info()}", file=sys.stderr, flush=True)
     elif tune:
         t0 = time.perf_counter()
         p = ctx.autotune()
-        print(f"[bench] prec={prec} autotuned plan {p} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+        setup["autotune_s"] = round(time.perf_counter() - t0, 4)
+        print(f"[bench] prec={prec} autotuned plan {p} ({setup['autotune_s']:.1f} s)", file=sys.stderr,
               flush=True)
     oc = opts.to_c()
     cap = (S + 1) * 64 * per_step
@@ -143,9 +150,11 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
         nn = np.array([pts[i][2] for i in idx], dtype=np.float32)
         return ctx.job_core_batch_raw(pm, mm, nn, oc, (S + 1) * per_step, cap)
 
+    t0 = time.perf_counter()
     for k in range(warmup):
         step(k)
     ctx.synchronize()
+    setup["warmup_steps_s"] = round(time.perf_counter() - t0, 4)  # the first call also allocates group buffers
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -170,7 +179,7 @@ def run_gpu(cfg, seg, prec, pts, warmup, steps, profile, batch=0, tune=True, pla
         ctx.synchronize()
         kt = ctx.kernel_times()
         ctx.profile(False)
-    return ctx, opts, nt, el, kt, ntrig, ntp
+    return ctx, opts, nt, el, kt, ntrig, ntp, setup
 
 
 def _match_triggers(r32, r64, thr, tol):
@@ -368,14 +377,14 @@ def main():
     if plan is not None and len(plan) not in (5, 6, 7):
         ap.error("--plan takes n1,e1,e2,fused,batch[,ntz[,stream]]")
     per_step = args.per_step or cfg["per_step"]
-    ctx32, opts, nt, el, kt, ntrig, ntp = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
+    ctx32, opts, nt, el, kt, ntrig, ntp, setup32 = run_gpu(cfg, seg, L.TDF_PREC_FP32, pts, args.warmup, args.steps, True,
                                              args.batch, not args.no_tune, plan, per_step)
     value = nt / el
     print(f"[bench] fp32: {nt} templates in {el:.3f} s -> {value:.0f} templates/s", file=sys.stderr, flush=True)
     fp64 = fperr = None
     if not args.no_fp64:
         k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 8)
-        ctx64, _, nt64, el64, kt64, _, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
+        ctx64, _, nt64, el64, kt64, _, _, _ = run_gpu(cfg, seg, L.TDF_PREC_FP64, pts, 1, k64, True, args.batch,
                                                 not args.no_tune, None, per_step)
         value64 = nt64 / el64
         print(f"[bench] fp64: {nt64} templates in {el64:.3f} s -> {value64:.0f} templates/s", file=sys.stderr,
@@ -443,6 +452,8 @@ def main():
         "memory_ceilings": ceil,
         "fp64_path": fp64,
         "fp32_vs_fp64": fperr,
+        "setup": dict(setup32, note="outside the timed steps: context creation with the segment upload "
+                                    "(host buffers, once per segment), autotune, warm-up calls"),
         "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},
         "kernels_ms_templates": ntp,
         "cpu_baseline": None,

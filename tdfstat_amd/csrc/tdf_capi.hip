@@ -1432,15 +1432,20 @@ int autotune_search(tdf_ctx *c, hipEvent_t e0, hipEvent_t e1, FftPlan &best, int
         best_t = std::min(best_t, cd.t);
     }
     // among the plans within 2 % of the fastest (the spread of these
-    // timings), the one with the most spindowns per launch: the work of a
-    // launch outside the two passes (trigger kernels, launch latency, the
-    // host's records) is then spread over more templates
+    // timings): the one with the most spindowns per launch (the work of a
+    // launch outside the two passes -- trigger kernels, launch latency, the
+    // host's records -- is then spread over more templates), then the
+    // largest pass-1 tile (longer runs of Z stores, no partly written
+    // lines), then the fastest
+    auto better = [&](const Cand &x, const Cand &y) {
+        if (x.b != y.b) return x.b > y.b;
+        if (x.p.e1 != y.p.e1) return x.p.e1 > y.p.e1;
+        return x.t < y.t;
+    };
     int best_k = -1;
     for (int k = 0; k < (int)cands.size(); k++) {
         if (cands[k].t > best_t * 1.02f) continue;
-        if (best_k < 0 || cands[k].b > cands[best_k].b ||
-            (cands[k].b == cands[best_k].b && cands[k].t < cands[best_k].t))
-            best_k = k;
+        if (best_k < 0 || better(cands[k], cands[best_k])) best_k = k;
     }
     if (best_k >= 0) {
         best = cands[best_k].p;
