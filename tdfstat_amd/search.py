@@ -13,6 +13,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib as L
+from . import inputs
 from . import settings as S
 
 
@@ -350,7 +351,8 @@ def run_info(sett, *, band=0, seg=1, hemi=0, thr=20.0, overlap=0.03125, narrowdo
     names = list(det_names) or [getattr(d, "name", "") for d in dets]
     for k, nm in enumerate(names[:8]):
         ri.det_name[k] = bs(nm)
-        ri.det_xdatname[k] = bs(f"{indir}/xdatc_{seg:03d}_{band:04d}.bin")
+        # the data file detectors_settings names (settings.c:143-154)
+        ri.det_xdatname[k] = bs(inputs.xdat_path(indir, seg, band, nm))
     ri._keep = keep
     return ri
 

@@ -133,3 +133,34 @@ def test_resume_inside_a_point_starts_at_the_saved_spindown(tmp_path):
     _same(got, want)
     assert not os.path.exists(_fname(tmp_path, 1))
     assert min(g.s for g in got if g.n == 0.0) >= 0
+
+
+def test_search_from_an_input_directory(tmp_path):
+    """A segment written in the reference's input directory layout and read
+    back by inputs.load_segment gives the same trigger files as the same
+    data searched from memory (sig2 from the data in both, as read_xdat
+    computes it from the file)."""
+    import copy
+
+    from tdfstat_amd import inputs as I
+    from tdfstat_amd import settings as S
+    _flush("search from an input directory")
+    seg = _segment()
+    indir = tmp_path / "in"
+    I.write_segment(seg, str(indir), 1, 372)
+    loaded = I.load_segment(str(indir), 1, 372, nod=2)
+    mem = copy.deepcopy(seg)
+    for d in mem.dets:
+        d.sig2 = S.xdat_sig2(d.xDat)[2]
+    opts = TS.default_opts(mem.sett, thr=6.0)
+    res = {}
+    for name, s in (("mem", mem), ("dir", loaded)):
+        out = tmp_path / name
+        out.mkdir()
+        r = _range(s.sett, mr=(-1, 0), pmr=(1, 1))
+        TS.read_checkpoint(r)
+        ri = TS.run_info(s.sett, band=372, seg=1, dets=s.dets, outdir=str(out), indir=str(indir), thr=6.0)
+        with TS.Context(s.sett, s.dets) as ctx:
+            res[name] = TS.search_file(ctx, ri, r, opts)
+    assert res["dir"] == res["mem"] and res["dir"][1] > 0
+    _same(TS.h5_read(_fname(tmp_path / "dir", 1)), TS.h5_read(_fname(tmp_path / "mem", 1)))
