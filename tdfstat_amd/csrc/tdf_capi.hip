@@ -1415,9 +1415,28 @@ int autotune_search(tdf_ctx *c, hipEvent_t e0, hipEvent_t e1, FftPlan &best, int
             }
     // second round: the best few plans over back-to-back pairs covering at
     // least 1024 templates, as a search runs them (a single pair also
-    // carries the launch gap and the ramp-up and drain of both kernels)
+    // carries the launch gap and the ramp-up and drain of both kernels).
+    // The four best plan shapes (factorisation, tiles, pass-2 form, Z
+    // access), each with its best batch and its largest one, so that the
+    // round is not filled with batch variants of one shape.
     std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.t < y.t; });
-    if (cands.size() > 6) cands.resize(6);
+    auto same_shape = [](const FftPlan &x, const FftPlan &y) {
+        return x.n1 == y.n1 && x.e1 == y.e1 && x.e2 == y.e2 && x.f2 == y.f2 && x.ntz == y.ntz;
+    };
+    std::vector<Cand> round2;
+    int nshape = 0;
+    for (size_t k = 0; k < cands.size() && nshape < 4; k++) {
+        bool seen = false;
+        for (size_t q = 0; q < k && !seen; q++) seen = same_shape(cands[q].p, cands[k].p);
+        if (seen) continue;
+        nshape++;
+        round2.push_back(cands[k]);  // the shape's best batch
+        const Cand *big = &cands[k];
+        for (const Cand &o2 : cands)
+            if (same_shape(o2.p, cands[k].p) && o2.b > big->b) big = &o2;
+        if (big->b != cands[k].b) round2.push_back(*big);
+    }
+    cands.swap(round2);
     for (Cand &cd : cands) {
         const SpinArgs sa = spin_args(cd.b);
         const int npairs = std::max(2, 1024 / cd.b);
