@@ -18,8 +18,9 @@ and against the ceiling of its access kind), spindown_pair (pass 1 + pass 2,
 a copy of the four-step intermediate through HBM, against the copy
 ceiling), fp64_path, fp32_vs_fp64 (matched-trigger, all-bin and absolute F
 errors, trigger-set equality), setup (context creation with the segment
-upload, autotune and warm-up times, outside the timed steps), kernels_ms and
-cpu_baseline.
+upload, autotune and warm-up times, outside the timed steps), kernels_ms,
+cpu_baseline (the oracle with the reference's PREC=FLOAT semantics) and
+cpu_baseline_fp64 (the FP64 oracle).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2day-2det|6day-3det|2day-2det-small]
   python bench.py --ceilings-only
@@ -255,11 +256,13 @@ def fp32_vs_fp64_error(cfg, seg, pts, opts, ctx32, ctx64, nsky=8, nss=8, thr_den
             "max_abs_dF_half": dabs, "trigger_sets_equal": bool(same)}
 
 
-def cpu_baseline(cfg, seg, pts, budget_s=12.0):
-    """Oracle job_core, reference PREC=FLOAT semantics, OpenMP over
-    spindowns on this host's cores; bounded sample of sky points."""
+def cpu_baseline(cfg, seg, pts, budget_s=12.0, fp64=False):
+    """Oracle job_core, reference PREC=FLOAT semantics (fp64: PREC=DOUBLE,
+    the FP64 oracle), OpenMP over spindowns on this host's cores; bounded
+    sample of sky points."""
     from oracle import oracle as O
     from tdfstat_amd import search as TS
+    prec = O.PREC_DOUBLE if fp64 else O.PREC_FLOAT
     sett = seg.sett
     S = cfg["nspin"]
     opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
@@ -269,11 +272,11 @@ def cpu_baseline(cfg, seg, pts, budget_s=12.0):
         ncpu = os.cpu_count() or 1
     threads = max(1, min(ncpu, 16))
     orc = O.Oracle(sett, seg.dets)
-    orc.job_core(*pts[0], opts, prec=O.PREC_FLOAT, iss_count=min(S, 2 * threads), nthreads=threads)  # warm
+    orc.job_core(*pts[0], opts, prec=prec, iss_count=min(S, 2 * threads), nthreads=threads)  # warm
     t0 = time.perf_counter()
     nt, nsky = 0, 0
     while True:
-        sky, n, _ = orc.job_core(*pts[nsky % len(pts)], opts, prec=O.PREC_FLOAT, nthreads=threads)
+        sky, n, _ = orc.job_core(*pts[nsky % len(pts)], opts, prec=prec, nthreads=threads)
         nt += sky.iss_size
         nsky += 1
         el = time.perf_counter() - t0
@@ -287,8 +290,8 @@ def cpu_baseline(cfg, seg, pts, budget_s=12.0):
         pass
     return dict(value=nt / el, unit="templates/s", cores=threads, kind="port", cpu_model=model,
                 sample=f"{nsky} sky point(s) x {S} spindowns ({nt} templates) of the same workload, "
-                       f"oracle job_core with PREC=FLOAT semantics, OpenMP over spindowns, "
-                       f"{threads} threads, {el:.1f} s")
+                       f"oracle job_core with PREC={'DOUBLE' if fp64 else 'FLOAT'} semantics, OpenMP over "
+                       f"spindowns, {threads} threads, {el:.1f} s")
 
 
 def plan_key(plan):
@@ -457,9 +460,11 @@ def main():
         "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},
         "kernels_ms_templates": ntp,
         "cpu_baseline": None,
+        "cpu_baseline_fp64": None,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, seg, pts)
+        line["cpu_baseline_fp64"] = cpu_baseline(cfg, seg, pts, budget_s=8.0, fp64=True)
     print(json.dumps(line), flush=True)
 
 
