@@ -1,0 +1,50 @@
+"""bench.py's bookkeeping on the CPU: the plan key it matches counter
+summaries by, and the committed rocprof HBM-traffic summaries, whose every
+variant must be found again by the lookup and agree with the algorithmic
+bytes of its kernel (pass 1 writes Fa, Fb: 16*nfftf per template, more only
+for partly written lines; pass 2 reads them back)."""
+import os
+
+import pytest
+
+import bench
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_plan_key_forms():
+    base = dict(n1=2048, e1=2048, e2=4096, fused=1, batch=256, ntz=1)
+    assert bench.plan_key(dict(base, stream=0)) == (2048, 2048, 4096, 1, 1)
+    assert bench.plan_key(dict(base, stream=1)) == (2048, 2048, 4096, 2, 1)
+    assert bench.plan_key(dict(base, stream=2)) == (2048, 2048, 4096, 3, 1)
+    assert bench.plan_key(dict(base, fused=0, stream=0, ntz=0)) == (2048, 2048, 4096, 0, 0)
+
+
+@pytest.mark.parametrize("config", ["2day-2det", "6day-3det"])
+def test_traffic_summaries_are_found_and_plausible(config):
+    path = os.path.join(ROOT, "profiles", f"spindown_kernels_{config}_hbm_traffic.txt")
+    lines = [ln for ln in open(path) if ln.startswith("variant ")]
+    assert len(lines) >= 2
+    for ln in lines:
+        kv = dict(t.split("=", 1) for t in ln.split()[1:])
+        n = int(kv["n"])
+        form = int(kv["form"])
+        plan = dict(n1=int(kv["n1"]), e1=int(kv["e1"]), e2=int(kv["e2"]), ntz=int(kv["ntz"]),
+                    fused=1 if form else 0, stream=form - 1 if form >= 2 else 0, batch=256)
+        got, src = bench.measured_traffic(config, kv["kernel"], plan, 10)
+        assert src.endswith(os.path.basename(path)), src
+        bpt = got / 10
+        alg = 16 * n  # Fa, Fb written by pass 1, read by pass 2
+        if kv["kernel"] == "spin_pass1_demod_fft":
+            # a pass-1 workgroup of TC columns stores runs of 16*TC bytes:
+            # exact from 128-byte runs on, partly written lines below
+            tc = int(kv["e1"]) // (2 * (n // int(kv["n1"])))
+            hi = 1.01 if 16 * tc >= 128 else 4.0
+            assert 0.99 * alg <= float(kv["write_per_template"]) <= hi * alg, ln
+        else:
+            assert 0.99 * alg <= float(kv["fetch_per_template"]) <= 1.02 * alg, ln
+        assert abs(bpt - float(kv["bytes_per_template"])) <= 1, ln
+    # an unprofiled plan is reported as such, not matched
+    got, why = bench.measured_traffic(config, "spin_pass1_demod_fft",
+                                      dict(n1=256, e1=512, e2=512, fused=0, stream=0, ntz=0, batch=8), 8)
+    assert got is None and "another plan" in why
