@@ -371,11 +371,27 @@ int check_opts(tdf_ctx *c, const tdf_job_opts *o)
 // Launches the spindown kernels for templates [t0, t0+count) of the current
 // call (descriptors already on the device): pass 1, pass 2, FStat blocks and
 // triggers for each batch of up to c->batch templates.  No host sync.
+// A second sky point launched together with the first (its whole range
+// after the first's): inputs, sky scalars, aa/bb, counters and trigger list.
+struct SecondPoint {
+    const void *xm;
+    const double *tmp1;
+    SkyScal sk;
+    const double *sky;
+    int *cnt;
+    int ncnt;  // counters cleared by pass 1
+    TrigEntry *trig;
+    int iss;
+};
+
+// pb: the whole ranges of two points in one launch (t0 = 0, count +
+// pb->iss <= batch, candidate path).
 template <typename T>
 int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, const SkyScal &sk,
                           bool use_cand, int *cnt_dev, int ncnt_tmpl, int nzero = 0, TrigEntry *trig_out = nullptr,
-                          long long trig_cap = 0)
+                          long long trig_cap = 0, const SecondPoint *pb = nullptr)
 {
+    if (pb && (t0 != 0 || count + pb->iss > c->batch || !use_cand)) return TDF_ERR_ARG;
     if (!trig_out) {
         trig_out = c->trig_d;
         trig_cap = c->trig_cap;
@@ -388,7 +404,7 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
     const int sparse = (use_cand && !c->keep_ft && p.f2 && p.e2 == 2 * p.n1) ? 1 : 0;
     int batch_idx = 0;
     for (int b0 = t0; b0 < t0 + count; b0 += c->batch, batch_idx++) {
-        const int nslots = std::min(c->batch, t0 + count - b0);
+        const int nslots = std::min(c->batch, t0 + count - b0) + (pb ? pb->iss : 0);
         int *cand_cnt = cnt_dev + 1 + ncnt_tmpl + batch_idx;
         SpinArgs sa{};
         sa.xm = c->xm;
@@ -412,6 +428,17 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
         sa.nzero = b0 == t0 ? nzero : 0;
         sa.cbits = c->cbits;
         sa.sparse = sparse;
+        sa.split = nslots;
+        if (pb) {
+            sa.split = count;
+            sa.xm1 = pb->xm;
+            sa.tmp1_1 = pb->tmp1;
+            sa.sk1 = pb->sk;
+            sa.sk1.iss0 = -count;  // slot s >= count is template s - count of the second point
+            sa.sky1 = pb->sky;
+            sa.zero_cnt1 = pb->cnt;
+            sa.nzero1 = pb->ncnt;
+        }
         {
             Prof pr(c, KS_SPIN1);
             HIPCHK(launch_spin_pass1<T>(p, sa, st));
@@ -445,6 +472,13 @@ int launch_spindown_range(tdf_ctx *c, int t0, int count, const tdf_job_opts *o, 
         ta.sk = sk;
         ta.cbits = c->cbits;
         ta.sparse = sparse;
+        ta.split = 0x7fffffff;
+        if (pb) {
+            ta.split = count;
+            ta.sk1 = sa.sk1;
+            ta.out1 = pb->trig;
+            ta.counters1 = pb->cnt;
+        }
         if (use_cand) {
             Prof pr(c, KS_TRIG);
             HIPCHK(launch_candidates<T>(c->FT, ta, c->cand_d, cand_cnt, (int)c->cand_cap, trig_out, cnt_dev, st));
@@ -597,7 +631,7 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
     c->prec = prec;
     c->csz = prec == TDF_PREC_FP64 ? 16 : 8;
     c->rsz = prec == TDF_PREC_FP64 ? 8 : 4;
-    c->batch = batch > 0 ? batch : (g->nfftf >= (1 << 19) ? 512 : 256);
+    c->batch = batch > 0 ? batch : 512;
     int rc = TDF_OK;
 #define TRY(x)                    \
     do {                          \
@@ -1017,6 +1051,7 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
     const int maxtrig = o->bufsize / 2;
     const long long prefix = std::min<long long>(c->trig_cap, BATCH_TRIG_PREFIX);
     std::vector<int> redo;
+    std::vector<int> paired_k(nsky, -1);  // point j launched with group point paired_k[j]
     bool overflow = false;
     long long np = 0;
     c->call_iss0 = 0;
@@ -1041,12 +1076,34 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
             use_point(c, k);
             const SkyScal sk{S[j].smin, o->sstep, mm[j], nn[j], S[j].het0, M[0], M[4], M[5], M[9], M[13], 0};
             const int nc = 1 + iss + (iss + c->batch - 1) / c->batch;
+            // the next point of the group joins this launch when both ranges
+            // fit it: half the launches (and their gaps) for short ranges
+            SecondPoint sp{};
+            const SecondPoint *pbp = nullptr;
+            if (use_cand && k + 1 < ng) {
+                const int j2 = grp(gi, k + 1), iss2 = S[j2].iss_size;
+                if (iss2 > 0 && iss + iss2 <= c->batch) {
+                    const tdf_geom &gg = c->g;
+                    sp.xm = (char *)c->xm_g + (size_t)(k + 1) * c->csz * 2 * gg.nifo * gg.nfftf;
+                    sp.tmp1 = c->tmp1_g + (size_t)(k + 1) * gg.nifo * gg.nfftf;
+                    sp.sk = SkyScal{S[j2].smin, o->sstep, mm[j2], nn[j2], S[j2].het0, M[0], M[4], M[5], M[9],
+                                    M[13], 0};
+                    sp.sky = stage_scal(c, c->stage_d) + 4 * (size_t)(k + 1);
+                    sp.cnt = stage_cnt(c, c->stage_d) + (size_t)(k + 1) * c->cnt_hb_cap;
+                    sp.ncnt = 1 + iss2 + (iss2 + c->batch - 1) / c->batch;
+                    sp.trig = stage_trig(c, c->stage_d) + (size_t)(k + 1) * BATCH_TRIG_PREFIX;
+                    sp.iss = iss2;
+                    pbp = &sp;
+                    paired_k[j2] = k;  // its candidates were counted in point k's counter
+                }
+            }
             // per-point counters and trigger list in the group's device area,
             // counters cleared by the first pass-1 launch (no memset call)
             r = launch_spindown_range<T>(c, 0, iss, o, sk, use_cand,
                                          stage_cnt(c, c->stage_d) + (size_t)k * c->cnt_hb_cap, iss, nc,
-                                         stage_trig(c, c->stage_d) + (size_t)k * BATCH_TRIG_PREFIX, prefix);
+                                         stage_trig(c, c->stage_d) + (size_t)k * BATCH_TRIG_PREFIX, prefix, pbp);
             if (r) return r;
+            if (pbp) k++;  // the second point is done
         }
         // aa/bb, counters and trigger lists of the group: one copy
         HIPCHK(hipMemcpyAsync(c->stage_h[slot], c->stage_d, c->stage_bytes, hipMemcpyDeviceToHost, c->stream));
@@ -1069,6 +1126,13 @@ int job_core_batch(tdf_ctx *c, int nsky, const int *pm, const float *mm, const f
             if (use_cand)
                 for (int b = 0; b < (iss + c->batch - 1) / c->batch; b++)
                     if (cnt[1 + iss + b] > c->cand_cap) again = true;
+            if (use_cand && paired_k[j] >= 0) {
+                // launched with the group's point paired_k[j]: one candidate
+                // list, counted in that point's first batch counter
+                const int kp = paired_k[j];
+                const int *cp = stage_cnt(c, c->stage_h[slot]) + (size_t)kp * c->cnt_hb_cap;
+                if (cp[1 + S[grp(gi, kp)].iss_size] > c->cand_cap) again = true;
+            }
             if (overflow) return TDF_OK;
             if (again) {
                 redo.push_back(j);

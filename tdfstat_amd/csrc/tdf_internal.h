@@ -165,6 +165,16 @@ struct SpinArgs {
     // F >= thr, with one bit per FT position in cbits ([slot][n/64])
     unsigned long long *cbits;
     int sparse;
+    // a second sky point in the same launch: slots >= split are its
+    // templates 0.. (split = nslots: one point), with its own inputs, sky
+    // scalars (iss0 = -split), aa/bb and counters
+    int split;
+    const void *xm1;
+    const double *tmp1_1;
+    SkyScal sk1;
+    const double *sky1;
+    int *zero_cnt1;
+    int nzero1;
 };
 
 struct FstatArgs {
@@ -180,6 +190,13 @@ struct TrigArgs {
     // sparse F: FT holds F only at the bins flagged in cbits (F >= thr)
     const unsigned long long *cbits;
     int sparse;
+    // launch slots >= split belong to a second sky point: its sky scalars
+    // (iss0 = -split), trigger list and counters (split = a large value:
+    // one point)
+    int split;
+    SkyScal sk1;
+    TrigEntry *out1;
+    int *counters1;
 };
 
 struct TrigEntry {

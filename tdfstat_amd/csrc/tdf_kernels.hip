@@ -270,13 +270,24 @@ struct IO_SpinP1 {
     int N, n, n1, nifo, slot0;
     int *zero_cnt;  // cleared by workgroup (0, 0) before the kernels that count
     int nzero;
+    // slots >= split: a second sky point (its inputs, scalars and counters)
+    int split;
+    const cx<T> *xm1;
+    const double *tmp1_1;
+    SkyScal sk1;
+    int *zero_cnt1;
+    int nzero1;
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
         constexpr int TC = NCOL / 2, NT = NCOL * L / 16, LP = ldsgeom<L>::LP;
-        if (nzero > 0 && blockIdx.x == 0 && blockIdx.y == 0)
+        if (blockIdx.x == 0 && blockIdx.y == 0) {
             for (int k = threadIdx.x; k < nzero; k += NT) zero_cnt[k] = 0;
-        const TmplDesc td = tmpl_scalars(sk, sk.iss0 + slot0 + blockIdx.y);
+            for (int k = threadIdx.x; k < nzero1; k += NT) zero_cnt1[k] = 0;
+        }
+        const bool second = (int)blockIdx.y >= split;
+        const SkyScal skp = second ? sk1 : sk;
+        const TmplDesc td = tmpl_scalars(skp, skp.iss0 + slot0 + blockIdx.y);
         const double hr = td.het1 * 0.15915494309189533577, sr = td.spnd * 0.15915494309189533577;
         const int c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
         const int nd = NIFO > 0 ? NIFO : nifo;
@@ -290,8 +301,8 @@ struct IO_SpinP1 {
         constexpr int NIT = TC * LQ / NT;
         constexpr bool WU = LQ % 64 == 0 && NT % 64 == 0;
         static_assert(TC * LQ % NT == 0, "tile");
-        const cx<T> *xmb = xm + (size_t)c0 * L;
-        const double *tmb = tmp1 + (size_t)c0 * L;
+        const cx<T> *xmb = (second ? xm1 : xm) + (size_t)c0 * L;
+        const double *tmb = (second ? tmp1_1 : tmp1) + (size_t)c0 * L;
         // FP32 bulk: the phase is formed directly in turns (het1/2pi,
         // spnd/2pi), reduced in FP64, sin/cos in hardware; FP64 path: the
         // phase in radians as the reference forms it
@@ -426,6 +437,10 @@ struct IO_SpinP2 {
     int cand_cap;
     unsigned long long *cbits;  // fused pass 2 with sparse F only
     int sparse;
+    int split;            // slots >= split: a second sky point,
+    const double *sky1;   // whose aa, bb these are
+    // aa, bb of the point that template slot belongs to
+    __device__ __forceinline__ const double *sky_of(int slot) const { return slot >= split ? sky1 : sky; }
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
@@ -448,7 +463,8 @@ struct IO_SpinP2 {
         const size_t base = (size_t)blockIdx.y * n;
         const int k20 = blockIdx.x * TC;
         // F = |Fa|^2/aa + |Fb|^2/bb with the two reciprocals formed once
-        const T raa = (T)1 / (T)sky[0], rbb = (T)1 / (T)sky[1];
+        const double *sk = sky_of(blockIdx.y);
+        const T raa = (T)1 / (T)sk[0], rbb = (T)1 / (T)sk[1];
         const T thrT = (T)thr;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int k1 = e % L, cl = e / L;
@@ -579,7 +595,8 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_fused(IO_SpinP2<T, NTZ> io
     __shared__ cx<T> tile[2 * ldsgeom<L>::LP];
     cx<T> v[16];
     pass2_load<T, L, NTZ>(io, blockIdx.y, blockIdx.x, v);
-    pass2_column<T, L, NTZ, LO>(io, twL, tile, blockIdx.y, blockIdx.x, v, (T)1 / (T)io.sky[0], (T)1 / (T)io.sky[1]);
+    const double *sk = io.sky_of(blockIdx.y);
+    pass2_column<T, L, NTZ, LO>(io, twL, tile, blockIdx.y, blockIdx.x, v, (T)1 / (T)sk[0], (T)1 / (T)sk[1]);
 }
 
 // Streaming form (FP32): a grid of resident workgroups walks the columns
@@ -629,7 +646,8 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_stream(IO_SpinP2<T, NTZ> i
     int item = blockIdx.x;
     if (item < nitems) stream_issue<T, L, NTZ>(io, item, raw);
     const int c = threadIdx.x & 1, j = threadIdx.x >> 1;
-    const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
+    const T raa0 = (T)1 / (T)io.sky[0], rbb0 = (T)1 / (T)io.sky[1];
+    const T raa1 = (T)1 / (T)io.sky1[0], rbb1 = (T)1 / (T)io.sky1[1];
     for (; item < nitems; item += gridDim.x) {
         // this wavefront's copies (and earlier stores) are done; the barrier
         // makes every wavefront's part visible
@@ -639,7 +657,10 @@ __global__ void __launch_bounds__(L / 8) k_spin_pass2_stream(IO_SpinP2<T, NTZ> i
 #pragma unroll
         for (int r = 0; r < 16; r++) v[r] = raw[2 * (j + r * (L / 16)) + c];
         const int nxt = item + gridDim.x;
-        pass2_column<T, L, NTZ, LO>(io, twL, tile, item / io.n2, item % io.n2, v, raa, rbb, [&]() {
+        const int slot = item / io.n2;
+        const bool second = slot >= io.split;
+        pass2_column<T, L, NTZ, LO>(io, twL, tile, slot, item % io.n2, v, second ? raa1 : raa0,
+                                    second ? rbb1 : rbb0, [&]() {
             // the staging area is read: the next copy may land
             if (nxt < nitems) stream_issue<T, L, NTZ>(io, nxt, raw);
         });
@@ -667,7 +688,8 @@ k_spin_pass2_stream_half(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL, in
         pass2_load<T, L, NTZ, 8, 16>(io, item / io.n2, item % io.n2, nv);
     }
     const int c = threadIdx.x & 1, j = threadIdx.x >> 1;
-    const T raa = (T)1 / (T)io.sky[0], rbb = (T)1 / (T)io.sky[1];
+    const T raa0 = (T)1 / (T)io.sky[0], rbb0 = (T)1 / (T)io.sky[1];
+    const T raa1 = (T)1 / (T)io.sky1[0], rbb1 = (T)1 / (T)io.sky1[1];
     for (; item < nitems; item += gridDim.x) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -677,8 +699,10 @@ k_spin_pass2_stream_half(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL, in
 #pragma unroll
         for (int r = 0; r < 8; r++) v[8 + r] = nv[r];
         const int nxt = item + gridDim.x;
+        const int slot = item / io.n2;
+        const bool second = slot >= io.split;
         pass2_column<T, L, NTZ, LO>(
-            io, twL, tile, item / io.n2, item % io.n2, v, raa, rbb,
+            io, twL, tile, slot, item % io.n2, v, second ? raa1 : raa0, second ? rbb1 : rbb0,
             [&]() {
                 if (nxt < nitems) stream_issue<T, L, NTZ, true>(io, nxt, raw);
             },
@@ -1060,7 +1084,8 @@ __global__ void __launch_bounds__(256) k_candidates(const T *__restrict__ FT, Tr
                     ig--;
                 }
                 if ((run & 1) == 0) {
-                    const double sgnl0 = tmpl_scalars(a.sk, a.sk.iss0 + gslot).sgnl0;
+                    const SkyScal &skp = gslot - a.slot0 >= a.split ? a.sk1 : a.sk;
+                    const double sgnl0 = tmpl_scalars(skp, skp.iss0 + gslot).sgnl0;
                     const double f_ = (double)(2 * i) / (double)a.nfftf * 3.14159265358979323846 + sgnl0;
                     bool veto = false;
                     for (int l = 0; l < a.numlines; l++)
@@ -1080,9 +1105,15 @@ __global__ void __launch_bounds__(256) k_candidates(const T *__restrict__ FT, Tr
         if (m) {
             const int lane = threadIdx.x & 63;
             if (emit) {
-                const int pos = atomicAdd(&counters[0], 1);
-                atomicAdd(&counters[1 + gslot], 1);
-                if (pos < a.cap) out[pos] = TrigEntry{gslot, i, fo, Fo};
+                // a second sky point's triggers go to its own list and
+                // counters, numbered from its first template
+                const bool second = gslot - a.slot0 >= a.split;
+                int *cnt = second ? a.counters1 : counters;
+                TrigEntry *dst = second ? a.out1 : out;
+                const int t = second ? gslot - a.slot0 - a.split : gslot;
+                const int pos = atomicAdd(&cnt[0], 1);
+                atomicAdd(&cnt[1 + t], 1);
+                if (pos < a.cap) dst[pos] = TrigEntry{t, i, fo, Fo};
             }
             (void)lane;
         }
@@ -1220,6 +1251,12 @@ static hipError_t launch_spin_pass1_n(const FftPlan &p, const SpinArgs &s, hipSt
     IO_SpinP1<T, NIFO, HALF, NTZ> io{(const cx<T> *)s.xm, s.tmp1, s.sk, (cx<T> *)s.Z,
                                      (const cx<T> *)p.tw_lo[FftCfg<T>::IS_D], (const cx<T> *)p.tw_hi[FftCfg<T>::IS_D],
                                      s.N, p.n, p.n1, s.nifo, s.slot0, s.zero_cnt, s.nzero};
+    io.split = s.split > 0 ? s.split : s.nslots;
+    io.xm1 = s.xm1 ? (const cx<T> *)s.xm1 : io.xm;
+    io.tmp1_1 = s.tmp1_1 ? s.tmp1_1 : io.tmp1;
+    io.sk1 = s.xm1 ? s.sk1 : s.sk;
+    io.zero_cnt1 = s.zero_cnt1;
+    io.nzero1 = s.zero_cnt1 ? s.nzero1 : 0;
     const int tc1 = p.e1 / p.n2 / 2;
     return launch_pass<T, -1>(p.n2, p.e1, io, dim3(p.n1 / tc1, s.nslots),
                               (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2], st);
@@ -1311,6 +1348,8 @@ static hipError_t launch_spin_pass2_z(const FftPlan &p, const SpinArgs &s, hipSt
     IO_SpinP2<T, NTZ> io{(const cx<T> *)s.Z, (T *)s.FT, s.sky, p.n, p.n1, p.n2, s.fr0, s.fr1,
                          s.emit_cand, s.slot0, s.thr, s.cand, s.cand_cnt, s.cand_cap, s.cbits,
                          (p.f2 && p.e2 == 2 * p.n1) ? s.sparse : 0};
+    io.split = s.split > 0 ? s.split : s.nslots;
+    io.sky1 = s.sky1 ? s.sky1 : s.sky;
     if (p.f2 && p.e2 == 2 * p.n1) {
         const cx<T> *tw = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
         const dim3 grid(p.n2, s.nslots);

@@ -89,7 +89,7 @@ class Context:
         self.prec = prec
         self._g = geom_from(sett)
         # spindowns per launch: enough workgroups to fill 256 CUs several times
-        self.batch = int(batch) if batch else (512 if sett.nfftf >= (1 << 19) else 256)
+        self.batch = int(batch) if batch else 512
         h = C.c_void_p()
         L.check(self.lib.tdf_ctx_create(C.byref(self._g), prec, self.batch, C.byref(h)), "tdf_ctx_create")
         self.h = h

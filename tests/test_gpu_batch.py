@@ -1,7 +1,8 @@
 """GPU: the pipelined multi-sky entry point (tdf_job_core_batch) returns the
 same skies, records and (f, F) pairs as one tdf_job_core call per point,
-including invisible points and points whose trigger lists overflow the
-pipelined read-back (synchronous fallback)."""
+including invisible points, points whose trigger lists overflow the
+pipelined read-back (synchronous fallback) and consecutive points whose two
+spindown ranges fit one launch (batch 24 against ranges of 9)."""
 import numpy as np
 import pytest
 
@@ -19,16 +20,17 @@ def _flush(msg):
 POINTS = [(1, 0.0, 0.0), (2, 1.0, -2.0), (1, 400.0, 400.0), (1, -3.0, 2.0), (2, 0.0, 5.0), (1, 2.0, 2.0)]
 
 
+@pytest.mark.parametrize("batch", [8, 24])
 @pytest.mark.parametrize("prec", [L.TDF_PREC_FP32, L.TDF_PREC_FP64])
 @pytest.mark.parametrize("thr", [20.0, 5.0, 2.0])
-def test_batch_equals_per_point(prec, thr):
-    _flush(f"batch prec={prec} thr={thr}")
+def test_batch_equals_per_point(prec, thr, batch):
+    _flush(f"batch prec={prec} thr={thr} batch={batch}")
     rng = np.random.default_rng(31)
     seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=31,
                              injections=[synth.random_injection(rng, 14.0) for _ in range(3)])
     sett = seg.sett
     opts = TS.default_opts(sett, thr=thr, sst=-4, spndr1=4, sstep=1)
-    with TS.Context(sett, seg.dets, prec=prec, batch=8) as ctx:
+    with TS.Context(sett, seg.dets, prec=prec, batch=batch) as ctx:
         want_sky, want = [], []
         for p in POINTS:
             sky, recs = ctx.job_core(*p, opts, pair_cap=1 << 21)
@@ -55,14 +57,16 @@ def test_batch_equals_per_point(prec, thr):
         assert ntrig > (1 << 16)  # more than the pipelined read-back holds for some points
 
 
+@pytest.mark.parametrize("batch", [8, 32])
 @pytest.mark.parametrize("prec", [L.TDF_PREC_FP32, L.TDF_PREC_FP64])
 @pytest.mark.parametrize("thr", [20.0, 5.0])
-def test_sparse_f_gives_the_same_records(prec, thr):
+def test_sparse_f_gives_the_same_records(prec, thr, batch):
     """Without kept F arrays the fused pass 2 writes F only at bins >= thr
     plus one flag bit per bin; the trigger rule then reads neighbours below
     thr as below the bin.  Records and (f, F) pairs are those of the full-F
-    path, bit for bit."""
-    _flush(f"sparse F prec={prec} thr={thr}")
+    path, bit for bit.  Batch 32 holds two ranges of 13 (two sky points per
+    launch on the sparse path, against one point per launch with kept F)."""
+    _flush(f"sparse F prec={prec} thr={thr} batch={batch}")
     rng = np.random.default_rng(37)
     seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=37,
                              injections=[synth.random_injection(rng, 14.0) for _ in range(3)])
@@ -77,8 +81,8 @@ def test_sparse_f_gives_the_same_records(prec, thr):
         _flush(f"  plan {plan}")
         out = []
         for keep in (True, False):
-            with TS.Context(sett, seg.dets, prec=prec, batch=8, keep_fstat=keep) as ctx:
-                ctx.set_plan(*plan[:4], 8, *plan[4:])
+            with TS.Context(sett, seg.dets, prec=prec, batch=batch, keep_fstat=keep) as ctx:
+                ctx.set_plan(*plan[:4], batch, *plan[4:])
                 out.append(ctx.job_core_batch(pts, opts, pair_cap=1 << 21))
         (_, n0, r0), (_, n1, r1) = out
         assert list(n0) == list(n1) and len(r0) == len(r1)
