@@ -105,7 +105,9 @@ const char *tdf_strerror(int code);
 int tdf_device_count(int *n);
 
 /* Replaces plan_fftw (init.c:710-755): creates the device context, its
- * FFT plans and work buffers.  batch = spindowns per launch (0 = auto). */
+ * FFT plans and work buffers.  batch = template slots per launch (0 = 512);
+ * tdf_job_core_batch launches two consecutive sky points together when
+ * their spindown ranges fit one batch. */
 int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out);
 int tdf_ctx_destroy(tdf_ctx *ctx);
 

@@ -88,7 +88,8 @@ class Context:
         self.sett = sett
         self.prec = prec
         self._g = geom_from(sett)
-        # spindowns per launch: enough workgroups to fill 256 CUs several times
+        # template slots per launch: enough workgroups to fill 256 CUs several
+        # times; 512 holds two 2-day sky points (255 spindowns each)
         self.batch = int(batch) if batch else 512
         h = C.c_void_p()
         L.check(self.lib.tdf_ctx_create(C.byref(self._g), prec, self.batch, C.byref(h)), "tdf_ctx_create")
