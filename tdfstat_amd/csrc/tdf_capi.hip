@@ -319,7 +319,8 @@ int run_sky_stage(tdf_ctx *c, const SkyBufs &b, int G, hipStream_t st)
     // forward FFT of length nfft, result column-major in XT
     {
         Prof pr(c, KS_SKY_FFT, st);
-        HIPCHK(launch_fft_natural_to_T<T>(c->p_fwd, b.xab, b.Zsky, b.XT, nifo * G, -1, 0, 0, 1.0, st));
+        HIPCHK(launch_fft_natural_to_T<T>(c->p_fwd, b.xab, b.Zsky, b.XT, nifo * G, -1, g.N, g.nfft, 1.0, g.nfft,
+                                          st));
     }
     // natural order + spectrum move into Ymv (jobcore.c:340-350)
     const int nyqst = g.nfft / 2 + 1;
@@ -329,17 +330,19 @@ int run_sky_stage(tdf_ctx *c, const SkyBufs &b, int G, hipStream_t st)
                                    g.Ninterp - g.nfft, g.nfft, st));
     }
     // inverse FFT of length Ninterp, scaled by interpftpad/Ninterp
+    // (only the bins up to the last spline node, nnodes, are computed out)
     const double ft = (double)g.interpftpad / g.Ninterp;
+    const int nnodes = g.interpftpad * g.N;
+    const int kmax = std::min(g.Ninterp, nnodes + 1);
     {
         Prof pr(c, KS_SKY_IFFT, st);
         HIPCHK(launch_fft_natural_to_T<T>(c->p_inv, b.Ymv, b.Zsky, b.XT, nifo * G, +1, nyqst,
-                                          nyqst + g.Ninterp - g.nfft, ft, st));
+                                          nyqst + g.Ninterp - g.nfft, ft, kmax, st));
     }
-    const int nnodes = g.interpftpad * g.N;
     {
         Prof pr(c, KS_SKY_TRANSPOSE2, st);
         HIPCHK(launch_transpose<T>(c->p_inv, b.XT, b.Ymv, 2 * nifo * G, (size_t)g.Ninterp, (size_t)g.Ninterp,
-                                   g.Ninterp, 0, std::min(g.Ninterp, nnodes + 1), st));
+                                   g.Ninterp, 0, kmax, st));
     }
     {
         Prof pr(c, KS_SPLINE, st);
@@ -670,7 +673,7 @@ int tdf_ctx_create(const tdf_geom *g, int prec, int batch, tdf_ctx **out)
     TRY(dmalloc(c, (void **)&c->shftf, sizeof(double) * nifo * (size_t)N));
     TRY(dmalloc(c, (void **)&c->tmp1, sizeof(double) * nifo * (size_t)g->nfftf));
     TRY(dmalloc(c, &c->xm, cs * 2 * nifo * (size_t)g->nfftf));
-    c->prep_nblk = (g->nfft + 255) / 256;
+    c->prep_nblk = (g->N + 255) / 256;
     TRY(dmalloc(c, (void **)&c->partials, sizeof(double) * 2 * nifo * (size_t)c->prep_nblk));
     TRY(dmalloc(c, (void **)&c->sky_scal, sizeof(double) * 4));
     TRY(dmalloc(c, (void **)&c->site_d, sizeof(SkySite) * (1 + 2 * GMAX_SKY)));
@@ -1352,11 +1355,11 @@ int tdf_fft_c2c(const double *in, double *out, int n, int howmany, int sign, int
     hipError_t e = hipMemcpy(din, h.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         if (prec == TDF_PREC_FP64) {
-            e = launch_fft_natural_to_T<double>(c->p_fwd, din, dtmp, dT, npairs, sign, 0, 0, 1.0, c->stream);
+            e = launch_fft_natural_to_T<double>(c->p_fwd, din, dtmp, dT, npairs, sign, 0, 0, 1.0, n, c->stream);
             if (e == hipSuccess)
                 e = launch_transpose<double>(c->p_fwd, dT, dout, 2 * npairs, n, n, n, 0, n, c->stream);
         } else {
-            e = launch_fft_natural_to_T<float>(c->p_fwd, din, dtmp, dT, npairs, sign, 0, 0, 1.0, c->stream);
+            e = launch_fft_natural_to_T<float>(c->p_fwd, din, dtmp, dT, npairs, sign, 0, 0, 1.0, n, c->stream);
             if (e == hipSuccess)
                 e = launch_transpose<float>(c->p_fwd, dT, dout, 2 * npairs, n, n, n, 0, n, c->stream);
         }

@@ -245,7 +245,7 @@ hipError_t launch_sky_reduce(const double *partials, int nblk, int nifo, const d
                              hipStream_t st);
 template <typename T>
 hipError_t launch_fft_natural_to_T(const FftPlan &p, const void *in, void *tmp, void *outT, int npairs, int sign,
-                                   int gap0, int gap1, double scale, hipStream_t st);
+                                   int gap0, int gap1, double scale, int kmax, hipStream_t st);
 template <typename T>
 hipError_t launch_transpose(const FftPlan &p, const void *inT, void *out, int nitems, size_t in_stride,
                             size_t out_stride, int split, int shift, int kmax, hipStream_t st);

@@ -47,8 +47,9 @@ __device__ __forceinline__ double phase0_of(double het0, int i, double oms, doub
     return het0 * (double)i + oms * shft;
 }
 
-// Per-sky prep: one thread per sample i < nfft of detector blockIdx.y of
-// sky point blockIdx.z of the group.
+// Per-sky prep: one thread per sample i < N of detector blockIdx.y of sky
+// point blockIdx.z of the group.  Rows N..nfft-1 of xab are never written:
+// the forward FFT reads them as zero.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restrict__ xab)
 {
@@ -59,30 +60,25 @@ __global__ void __launch_bounds__(256) k_sky_prep(SkyPrepArgs a, cx<T> *__restri
     const size_t nfftf = (size_t)a.lay.n1 * a.lay.n2;
     xab += (size_t)g * 2 * nifo * a.nfft;
     double a2 = 0., b2 = 0.;
-    if (i < a.nfft) {
-        cx<T> xa = {(T)0, (T)0}, xb = {(T)0, (T)0};
-        if (i < a.N) {
-            const float x = a.xDat[(size_t)d * a.N + i];
-            const double *ssb0 = a.DetSSB + (size_t)d * 3 * a.N;
-            const double shft = shft_dot(ss.ns0, ss.ns1, ss.ns2, ssb0 + 3 * (size_t)i);
-            const double shftf = shft - shft_dot(ss.ns0, ss.ns1, ss.ns2, ssb0);
-            const double tmp1 = tmp1_of(i, shft);
-            a.shftf[((size_t)g * nifo + d) * a.N + i] = shftf;
-            a.tmp1[((size_t)g * nifo + d) * nfftf + a.lay.at(i)] = tmp1;
-            double aa, bb;
-            const ModvirSky ms{ss.c2d, ss.c2sd, ss.sindel, ss.cosdel, ss.cosalfr[d], ss.sinalfr[d]};
-            modvir_sample(dp.c, ms, a.omr, i, x, &aa, &bb);
-            a2 = aa * aa;
-            b2 = bb * bb;
-            const double phase = phase0_of(ss.het0, i, a.oms, shft);
-            T sp, cp;
-            sincos_fp64arg(phase, &sp, &cp);
-            const T xaa = (T)((double)x * aa), xbb = (T)((double)x * bb);
-            xa = cx<T>{xaa * cp, -xaa * sp};
-            xb = cx<T>{xbb * cp, -xbb * sp};
-        }
-        xab[(size_t)(2 * d) * a.nfft + i] = xa;
-        xab[(size_t)(2 * d + 1) * a.nfft + i] = xb;
+    if (i < a.N) {
+        const float x = a.xDat[(size_t)d * a.N + i];
+        const double *ssb0 = a.DetSSB + (size_t)d * 3 * a.N;
+        const double shft = shft_dot(ss.ns0, ss.ns1, ss.ns2, ssb0 + 3 * (size_t)i);
+        const double shftf = shft - shft_dot(ss.ns0, ss.ns1, ss.ns2, ssb0);
+        const double tmp1 = tmp1_of(i, shft);
+        a.shftf[((size_t)g * nifo + d) * a.N + i] = shftf;
+        a.tmp1[((size_t)g * nifo + d) * nfftf + a.lay.at(i)] = tmp1;
+        double aa, bb;
+        const ModvirSky ms{ss.c2d, ss.c2sd, ss.sindel, ss.cosdel, ss.cosalfr[d], ss.sinalfr[d]};
+        modvir_sample(dp.c, ms, a.omr, i, x, &aa, &bb);
+        a2 = aa * aa;
+        b2 = bb * bb;
+        const double phase = phase0_of(ss.het0, i, a.oms, shft);
+        T sp, cp;
+        sincos_fp64arg(phase, &sp, &cp);
+        const T xaa = (T)((double)x * aa), xbb = (T)((double)x * bb);
+        xab[(size_t)(2 * d) * a.nfft + i] = cx<T>{xaa * cp, -xaa * sp};
+        xab[(size_t)(2 * d + 1) * a.nfft + i] = cx<T>{xbb * cp, -xbb * sp};
     }
     // block reduction of aa^2, bb^2 (fixed order)
     __shared__ double ra[256], rb[256];
@@ -218,7 +214,7 @@ struct IO_P1Nat {
 };
 
 // Pass 2 writing the column-major ("transposed") result:
-// out[k2*n1 + k1] = X[k2 + n2*k1] * scale.
+// out[k2*n1 + k1] = X[k2 + n2*k1] * scale, only for bins k2 + n2*k1 < kmax.
 template <typename T>
 struct IO_P2T {
     static constexpr bool HALF = false;
@@ -226,6 +222,7 @@ struct IO_P2T {
     cx<T> *out;
     int n, n1;
     T scale;
+    int n2, kmax;
     template <int SIGN, int L, int NCOL>
     __device__ __forceinline__ void load(cx<T> *tile)
     {
@@ -247,6 +244,7 @@ struct IO_P2T {
         const int k20 = blockIdx.x * TC;
         for (int e = threadIdx.x; e < TC * L; e += NT) {
             const int k1 = e % L, cl = e / L;
+            if (k20 + cl + n2 * k1 >= kmax) continue;
             const size_t off = (size_t)(k20 + cl) * n1 + k1;
             out[base + off] = cscale(tile[cl * LP + ldsgeom<L>::pad(k1)], scale);
             out[base + n + off] = cscale(tile[(TC + cl) * LP + ldsgeom<L>::pad(k1)], scale);
@@ -716,7 +714,7 @@ k_spin_pass2_stream_half(IO_SpinP2<T, NTZ> io, const cx<T> *__restrict__ twL, in
 // Batched transpose of column-major FFT output: in[item][k2*n1 + k1]
 // (rows k2 < n2, cols k1 < n1) -> natural bin k = k2 + n2*k1, written to
 // out[item][dst(k)], dst(k) = k for k < split, k + shift otherwise; only
-// k < kmax are written.
+// k < kmax are read and written.
 template <typename T>
 __global__ void __launch_bounds__(256) k_transpose(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                                    int n1, int n2, size_t in_stride, size_t out_stride,
@@ -730,7 +728,7 @@ __global__ void __launch_bounds__(256) k_transpose(const cx<T> *__restrict__ in,
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int r = ty; r < 32; r += 8) {
         const int k2 = k2b + r, k1 = k1b + tx;
-        if (k2 < n2 && k1 < n1) t[r][tx] = src[(size_t)k2 * n1 + k1];
+        if (k2 < n2 && k1 < n1 && k2 + n2 * k1 < kmax) t[r][tx] = src[(size_t)k2 * n1 + k1];
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
@@ -1197,7 +1195,7 @@ hipError_t launch_sky_reduce(const double *partials, int nblk, int nifo, const d
 // two-pass FFT on nitems pairs of natural-order arrays; result column-major
 template <typename T>
 hipError_t launch_fft_natural_to_T(const FftPlan &p, const void *in, void *tmp, void *outT, int npairs, int sign,
-                                   int gap0, int gap1, double scale, hipStream_t st)
+                                   int gap0, int gap1, double scale, int kmax, hipStream_t st)
 {
     const cx<T> *twL1 = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l2]; // pass-1 length n2
     const cx<T> *twL2 = (const cx<T> *)p.twL[FftCfg<T>::IS_D][p.l1];
@@ -1209,7 +1207,7 @@ hipError_t launch_fft_natural_to_T(const FftPlan &p, const void *in, void *tmp, 
     if (sign < 0) e = launch_pass<T, -1>(p.n2, p.e1, io1, dim3(p.n1 / tc1, npairs), twL1, st);
     else e = launch_pass<T, +1>(p.n2, p.e1, io1, dim3(p.n1 / tc1, npairs), twL1, st);
     if (e != hipSuccess) return e;
-    IO_P2T<T> io2{(const cx<T> *)tmp, (cx<T> *)outT, p.n, p.n1, (T)scale};
+    IO_P2T<T> io2{(const cx<T> *)tmp, (cx<T> *)outT, p.n, p.n1, (T)scale, p.n2, kmax};
     if (sign < 0) e = launch_pass<T, -1>(p.n1, p.e2, io2, dim3(p.n2 / tc2, npairs), twL2, st);
     else e = launch_pass<T, +1>(p.n1, p.e2, io2, dim3(p.n2 / tc2, npairs), twL2, st);
     return e;
@@ -1411,7 +1409,7 @@ hipError_t launch_candidates(const void *FT, const TrigArgs &a, const TrigEntry 
 #define TDF_INST(T)                                                                                        \
     template hipError_t launch_sky_prep<T>(const SkyPrepArgs &, void *, int, int, int, hipStream_t);      \
     template hipError_t launch_fft_natural_to_T<T>(const FftPlan &, const void *, void *, void *, int, int, \
-                                                   int, int, double, hipStream_t);                         \
+                                                   int, int, double, int, hipStream_t);                    \
     template hipError_t launch_transpose<T>(const FftPlan &, const void *, void *, int, size_t, size_t, int,  \
                                             int, int, hipStream_t);                                        \
     template hipError_t launch_spline<T>(const void *, void *, int, size_t, size_t, int, const double *,     \
