@@ -22,10 +22,20 @@ upload, autotune and warm-up times, outside the timed steps), kernels_ms,
 cpu_baseline (the oracle with the reference's PREC=FLOAT semantics) and
 cpu_baseline_fp64 (the FP64 oracle).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2day-2det|6day-3det|2day-2det-small]
+--config segments-3det runs 32 uneven 3-detector segments (2, 4 or 6 days,
+gaps and sigma drawn per detector), one resident context each; a step is the
+whole search of one segment (255 sky x 63 spindowns), the default steps cover
+every segment once, and the roofline, passes and pair figures sum each
+context's algorithmic bytes over its own HIP-event kernel times.  Its
+cpu_baseline times the oracle on each segment length and weights the rates
+by the timed template mix; fp32_vs_fp64 is taken on one 6-day segment.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--config 2day-2det|6day-3det|2day-2det-small|segments-3det]
   python bench.py --ceilings-only
 """
 import argparse
+import collections
 import copy
 import json
 import math
@@ -53,6 +63,13 @@ CONFIGS = {
     # 2 detectors, 2 days, 3x5 sky x 33 spindowns, one injection
     "2day-2det-small": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(3, 5), sky_step=1.0, nspin=33,
                injections=1, gaps=None, sigmas=None, per_step=15),
+    # 32 uneven segments, 3 detectors: 2, 4 or 6 days each (nfftf 2^17, 2^18,
+    # 2^19 with fftpad 2 at 6 days), gaps over U[0.1, 0.3] and sigma
+    # U[0.8, 1.5] per detector, one injection each; 15x17 sky x 63 spindowns
+    # per segment; one context per segment, all resident; a step is the
+    # whole search of one segment
+    "segments-3det": dict(dets=("H1", "L1", "V1"), segments=32, nods=(2, 4, 6), sky=(15, 17), sky_step=0.3,
+               nspin=63, batch=128, per_step=32),
 }
 
 
@@ -82,6 +99,24 @@ def make_workload(name, seed=20261203):
     seg = synth.make_segment(nod=cfg["nod"], det_names=cfg["dets"], fftpad=cfg["fftpad"], seed=seed,
                              gap_fracs=cfg["gaps"], sigmas=cfg["sigmas"], injections=injs)
     return cfg, seg
+
+
+def make_segments(cfg, seed=20261203):
+    """Seeded uneven segments: length drawn from cfg["nods"], gap fraction
+    U[0.1, 0.3] and sigma U[0.8, 1.5] per detector, one injection at SNR
+    U[8, 15] each."""
+    from tdfstat_amd import synth
+    rng = np.random.default_rng(seed)
+    nd = len(cfg["dets"])
+    segs = []
+    for s in range(cfg["segments"]):
+        nod = int(rng.choice(cfg["nods"]))
+        gaps = tuple(float(g) for g in rng.uniform(0.1, 0.3, nd))
+        sig = tuple(float(v) for v in rng.uniform(0.8, 1.5, nd))
+        inj = synth.random_injection(rng, float(rng.uniform(8, 15)))
+        segs.append(synth.make_segment(nod=nod, det_names=cfg["dets"], fftpad=2 if nod == 6 else 1,
+                                       seed=seed + 1 + s, gap_fracs=gaps, sigmas=sig, injections=[inj]))
+    return segs
 
 
 def algorithmic_bytes(sett, S, prec64):
@@ -321,11 +356,170 @@ def measured_traffic(config, kname, plan, nslots):
                 return int(float(kv["bytes_per_template"]) * nslots), os.path.relpath(path, ROOT)
     return None, "counter summary taken under another plan"
 
+def run_segments(args, ceil):
+    """The uneven-segments workload: one FP32 context per segment (all
+    resident, each segment's data uploaded once), the plan tuned once per
+    segment geometry (N, nfftf) and set on the others, every context warmed
+    by one call (group buffers), then `steps` steps, step k being the search
+    of segment (warmup + k) mod nseg over all its sky points in
+    tdf_job_core_batch calls of per_step points.  Kernel times come from a
+    repetition of one call per context with HIP events."""
+    from tdfstat_amd import _lib as L
+    from tdfstat_amd import search as TS
+    import torch
+    cfg = CONFIGS[args.config]
+    S, per_step = cfg["nspin"], args.per_step or cfg["per_step"]
+    t0 = time.perf_counter()
+    segs = make_segments(cfg)
+    setup = {"segment_generation_s": round(time.perf_counter() - t0, 3)}
+    t0 = time.perf_counter()
+    plans, runs = {}, []
+    for seg in segs:
+        sett = seg.sett
+        pts = sky_points(cfg, sett)
+        opts = TS.default_opts(sett, thr=20.0, sst=-(S // 2), spndr1=S // 2, sstep=1)
+        ctx = TS.Context(sett, seg.dets, prec=L.TDF_PREC_FP32, batch=args.batch or cfg["batch"], keep_fstat=False)
+        key = (sett.N, sett.nfftf)
+        if key not in plans:
+            p = ctx.autotune()
+            plans[key] = p
+            print(f"[bench] N={sett.N} nfftf={sett.nfftf} autotuned plan {p}", file=sys.stderr, flush=True)
+        else:
+            p = plans[key]
+            ctx.set_plan(p["n1"], p["e1"], p["e2"], p["fused"], p["batch"], p["ntz"], p["stream"])
+        chunks = []
+        for j in range(0, len(pts), per_step):
+            sel = pts[j:j + per_step]
+            chunks.append((np.array([q[0] for q in sel], dtype=np.int32), np.array([q[1] for q in sel], dtype=np.float32),
+                           np.array([q[2] for q in sel], dtype=np.float32)))
+        runs.append(dict(seg=seg, ctx=ctx, pts=pts, opts=opts, oc=opts.to_c(), chunks=chunks))
+    setup["contexts_upload_autotune_s"] = round(time.perf_counter() - t0, 3)
+    cap = (S + 1) * 64 * per_step
+
+    def call(r, ch):
+        return r["ctx"].job_core_batch_raw(*ch, r["oc"], (S + 1) * per_step, cap)
+
+    nseg = len(runs)
+    t0 = time.perf_counter()
+    for r in runs:
+        call(r, r["chunks"][0])
+    for k in range(args.warmup):
+        for ch in runs[k % nseg]["chunks"]:
+            call(runs[k % nseg], ch)
+    for r in runs:
+        r["ctx"].synchronize()
+    setup["warmup_s"] = round(time.perf_counter() - t0, 3)
+    torch.cuda.synchronize()
+    nt = ntrig = 0
+    per_seg = [0] * nseg
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        i = (args.warmup + k) % nseg
+        for ch in runs[i]["chunks"]:
+            a, b = call(runs[i], ch)
+            nt += a
+            ntrig += b
+            per_seg[i] += a
+    for r in runs:
+        r["ctx"].synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = nt / el
+    print(f"[bench] {args.config}: {nt} templates over {args.steps} segment searches in {el:.3f} s -> "
+          f"{value:.0f} templates/s", file=sys.stderr, flush=True)
+
+    # kernel times: one call per context with HIP events; algorithmic bytes
+    # summed per context (its own N, nfftf, slots per launch)
+    kb = {k: [0.0, 0.0, 0] for k in ("spin_pass1_demod_fft", "spin_pass2_fft_fstat", "triggers")}
+    for r in runs:
+        r["ctx"].profile(True, spindown_only=True)
+        ntp = call(r, r["chunks"][0])[0]
+        r["ctx"].synchronize()
+        kt = r["ctx"].kernel_times()
+        r["ctx"].profile(False)
+        nsl = ntp / max(1, kt.get("spin_pass1_demod_fft", (0, 1))[1])
+        for k in kb:
+            if k in kt:
+                kb[k][0] += kernel_bytes(r["seg"].sett, S, nsl, k, False) * kt[k][1]
+                kb[k][1] += kt[k][0]
+                kb[k][2] += kt[k][1]
+    dom = max(kb, key=lambda k: kb[k][1])
+    ach = kb[dom][0] / (kb[dom][1] * 1e-3) / 1e9
+    passes = {}
+    for k, kind in (("spin_pass1_demod_fft", "write"), ("spin_pass2_fft_fstat", "read")):
+        a_ = kb[k][0] / (kb[k][1] * 1e-3) / 1e9
+        passes[k] = {"achieved": round(a_, 1), "frac": round(a_ / HBM_PEAK_GBS, 4), "ceiling_kind": kind,
+                     "frac_of_ceiling": round(a_ / ceil[kind], 4) if ceil else None}
+    b_pair = kb["spin_pass1_demod_fft"][0] + kb["spin_pass2_fft_fstat"][0]
+    t_pair = (kb["spin_pass1_demod_fft"][1] + kb["spin_pass2_fft_fstat"][1]) * 1e-3
+    pair = {"achieved": round(b_pair / t_pair / 1e9, 1), "frac": round(b_pair / t_pair / 1e9 / HBM_PEAK_GBS, 4),
+            "ceiling_kind": "copy", "frac_of_ceiling": round(b_pair / t_pair / 1e9 / ceil["copy"], 4) if ceil else None}
+    pipe = sum(per_seg[i] * algorithmic_bytes(runs[i]["seg"].sett, S, False) for i in range(nseg)) / el / 1e9
+    lens = collections.Counter(r["seg"].sett.N for r in runs)
+
+    # FP32 vs FP64 on the first segment of the longest length
+    r6 = max(runs, key=lambda r: r["seg"].sett.N)
+    with TS.Context(r6["seg"].sett, r6["seg"].dets, prec=L.TDF_PREC_FP64, batch=args.batch or cfg["batch"]) as c64:
+        c64.autotune()
+        fperr = fp32_vs_fp64_error(cfg, r6["seg"], r6["pts"], r6["opts"], r6["ctx"], c64, nsky=4)
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on a bounded sample of each segment length, combined
+        # with the template mix of the timed steps
+        rates, samples = {}, []
+        for N in sorted(lens):
+            r = next(r for r in runs if r["seg"].sett.N == N)
+            cb = cpu_baseline(cfg, r["seg"], r["pts"], budget_s=4.0)
+            rates[N] = cb["value"]
+            samples.append(f"N={N}: {cb['sample']}")
+        t_cpu = sum(per_seg[i] / rates[runs[i]["seg"].sett.N] for i in range(nseg))
+        cpu = dict(value=nt / t_cpu, unit="templates/s", cores=cb["cores"], kind="port", cpu_model=cb["cpu_model"],
+                   sample="; ".join(samples) + "; rates combined with the timed template mix")
+    for r in runs:
+        r["ctx"].close()
+    mix = ", ".join(f"{lens[N]} x N={N}" for N in sorted(lens))
+    line = {
+        "metric": "templates/s (sky x spindown), FP32 bulk + FP64 phase",
+        "value": round(value, 1),
+        "unit": "templates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {nseg} segments, nifo=3 ({mix}), gaps 10-30 %, sigma 0.8-1.5, "
+                               f"{cfg['sky'][0]}x{cfg['sky'][1]} sky x {S} spindowns each, seeded Gaussian noise + "
+                               f"one injection per segment, FP64 phase",
+                   "templates_timed": nt, "triggers": ntrig,
+                   "plans": {f"N={k[0]} nfftf={k[1]}": v for k, v in plans.items()},
+                   "sky_points_per_call": per_step},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "avg_launch_ms": round(kb[dom][1] / kb[dom][2], 5),
+                     "bytes_per_launch": int(kb[dom][0] / kb[dom][2]),
+                     "traffic_source": "no counter summary for this workload (launch sizes differ per segment)"},
+        "pipeline_roofline": {"achieved": round(pipe, 1), "frac": round(pipe / HBM_PEAK_GBS, 4)},
+        "passes": passes,
+        "spindown_pair": pair,
+        "memory_ceilings": ceil,
+        "fp64_path": None,
+        "fp32_vs_fp64": dict(fperr, segment=f"N={r6['seg'].sett.N} nfftf={r6['seg'].sett.nfftf}"),
+        "setup": dict(setup, note="outside the timed steps"),
+        "kernels_ms": {k: [round(v[1], 3), v[2]] for k, v in kb.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 200; segments-3det: one search per segment)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="2day-2det", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
@@ -343,6 +537,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a plain 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = CONFIGS[args.config].get("segments", 200)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(the search shards by sky point across GPUs, see DESIGN.md).", file=sys.stderr)
@@ -369,6 +565,9 @@ def main():
             print(json.dumps({"memory_ceilings": ceil}), flush=True)
             return
 
+    if "segments" in CONFIGS[args.config]:
+        run_segments(args, ceil)
+        return
     cfg, seg = make_workload(args.config)
     sett = seg.sett
     pts = sky_points(cfg, sett)

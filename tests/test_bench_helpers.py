@@ -48,3 +48,26 @@ def test_traffic_summaries_are_found_and_plausible(config):
     got, why = bench.measured_traffic(config, "spin_pass1_demod_fft",
                                       dict(n1=256, e1=512, e2=512, fused=0, stream=0, ntz=0, batch=8), 8)
     assert got is None and "another plan" in why
+
+
+def test_uneven_segments_workload():
+    """The uneven-segments workload: seeded (the same segments twice), every
+    length drawn from 2/4/6 days with its nfft (fftpad 2 at 6 days), gap
+    fractions in [0.1, 0.3] (whole gaps of N/50 samples may overshoot by
+    one gap) and sigma in [0.8, 1.5] per detector, and the whole sky grid
+    visible."""
+    import numpy as np
+
+    cfg = dict(bench.CONFIGS["segments-3det"], segments=6)
+    a, b = bench.make_segments(cfg), bench.make_segments(cfg)
+    lens = {86164: (131072, 131072), 172328: (262144, 262144), 258492: (262144, 524288)}
+    for s, t in zip(a, b):
+        assert s.sett.N == t.sett.N
+        np.testing.assert_array_equal(s.dets[0].xDat, t.dets[0].xDat)
+        assert (s.sett.nfft, s.sett.nfftf) == lens[s.sett.N]
+        assert [d.name for d in s.dets] == ["H1", "L1", "V1"]
+        for d in s.dets:
+            gap = float(np.mean(d.xDat == 0))
+            assert 0.1 <= gap <= 0.3 + 1.0 / 50 + 1e-9, gap
+            assert 0.8 <= d.sigma <= 1.5
+        assert len(bench.sky_points(cfg, s.sett)) == cfg["sky"][0] * cfg["sky"][1]
