@@ -55,7 +55,7 @@ PROFILED_STEPS = 40
 CONFIGS = {
     # 2 detectors, 2 days (N=86164), nfft=2^17, 31x33 sky x 255 spindowns
     "2day-2det": dict(dets=("H1", "L1"), nod=2, fftpad=1, sky=(31, 33), sky_step=0.15, nspin=255,
-               injections=4, gaps=None, sigmas=None, per_step=32),
+               injections=4, gaps=None, sigmas=None, per_step=64),
     # 3 detectors, 6 days (N=258492), nfft=2^18, fftpad=2 -> nfftf=2^19,
     # 99x101 sky x 1023 spindowns, 20% gaps
     "6day-3det": dict(dets=("H1", "L1", "V1"), nod=6, fftpad=2, sky=(99, 101), sky_step=0.045, nspin=1023,
@@ -69,7 +69,7 @@ CONFIGS = {
     # per segment; one context per segment, all resident; a step is the
     # whole search of one segment
     "segments-3det": dict(dets=("H1", "L1", "V1"), segments=32, nods=(2, 4, 6), sky=(15, 17), sky_step=0.3,
-               nspin=63, batch=128, per_step=32),
+               nspin=63, batch=128, per_step=255),
 }
 
 
