@@ -65,7 +65,7 @@ def test_sparse_f_gives_the_same_records(prec, thr, batch):
     plus one flag bit per bin; the trigger rule then reads neighbours below
     thr as below the bin.  Records and (f, F) pairs are those of the full-F
     path, bit for bit.  Batch 32 holds two ranges of 13 (two sky points per
-    launch on the sparse path, against one point per launch with kept F)."""
+    launch, with and without kept F)."""
     _flush(f"sparse F prec={prec} thr={thr} batch={batch}")
     rng = np.random.default_rng(37)
     seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=37,
