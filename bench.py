@@ -541,7 +541,8 @@ def main():
         args.steps = CONFIGS[args.config].get("segments", 200)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the search shards by sky point across GPUs, see DESIGN.md).", file=sys.stderr)
+              f"(a multi-GPU search would give each GPU its own rows of the sky grid; nothing is "
+              f"exchanged on the hot path).", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no GPU"

@@ -92,7 +92,7 @@ def test_checkpoint_resume_gives_the_same_files(tmp_path):
         with pytest.raises(L.TdfError):  # and a restart then has nothing to do
             TS.read_checkpoint(_range(sett), st_full, 1)
         # an interrupted run: hemisphere 1, rows mm = -1, 0 done, then the
-        # state the driver writes after the row mm = 0
+        # state tdf_search writes after the row mm = 0
         ri_p = TS.run_info(sett, band=372, seg=1, dets=seg.dets, outdir=str(part), checkp_flag=1)
         r = _range(sett, mr=(-1, 0), pmr=(1, 1))
         TS.read_checkpoint(r)
