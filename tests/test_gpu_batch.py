@@ -90,3 +90,36 @@ def test_sparse_f_gives_the_same_records(prec, thr, batch):
             assert (a.m, a.n, a.s) == (b.m, b.n, b.s)
             np.testing.assert_array_equal(a.ffstat, b.ffstat)
         _flush(f"  {sum(len(r.ffstat) for r in r0)} triggers")
+
+
+@pytest.mark.parametrize("prec", [L.TDF_PREC_FP32, L.TDF_PREC_FP64])
+def test_shared_launch_veto_lines_equal_per_point(prec):
+    """Sky points sharing a launch veto with their own sgnl0: interior veto
+    lines over part of the band, batch 24 (two ranges of 9 per launch),
+    records and (f, F) pairs equal to one tdf_job_core call per point, and
+    some triggers vetoed."""
+    _flush(f"shared launch veto prec={prec}")
+    rng = np.random.default_rng(41)
+    seg = synth.make_segment(nod=2, det_names=("H1", "L1"), seed=41,
+                             injections=[synth.random_injection(rng, 14.0) for _ in range(3)])
+    sett = seg.sett
+    plain = TS.default_opts(sett, thr=5.0, sst=-4, spndr1=4, sstep=1)
+    opts = TS.default_opts(sett, thr=5.0, sst=-4, spndr1=4, sstep=1)
+    opts.lines = tuple(plain.lines) + tuple((float(a), float(a) + 0.04) for a in np.linspace(0.4, 2.6, 15))
+    pts = POINTS[:2] + POINTS[3:]
+    with TS.Context(sett, seg.dets, prec=prec, batch=24) as ctx:
+        want = [ctx.job_core(*p, opts, pair_cap=1 << 21)[1] for p in pts]
+        n_plain = sum(len(r.ffstat) for p in pts for r in ctx.job_core(*p, plain, pair_cap=1 << 21)[1])
+        _, nper, recs = ctx.job_core_batch(pts, opts, pair_cap=1 << 21)
+    k = 0
+    for j, wr in enumerate(want):
+        assert nper[j] == len(wr)
+        for r in wr:
+            g = recs[k]
+            k += 1
+            assert (g.m, g.n, g.s) == (r.m, r.n, r.s)
+            np.testing.assert_array_equal(g.ffstat, r.ffstat)
+    assert k == len(recs)
+    ntrig = sum(len(r.ffstat) for r in recs)
+    _flush(f"  {ntrig} triggers with the lines, {n_plain} without")
+    assert 0 < ntrig < n_plain
